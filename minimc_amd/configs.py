@@ -1,0 +1,77 @@
+"""The benchmark problems, built as `Problem`s with synthetic data.
+
+Each builder documents its geometry, data and seeds.  Energies in MeV,
+lengths in cm, cross sections in barns (atoms/(barn cm) for densities).
+"""
+import numpy as np
+
+from . import abi
+from .world import Problem, ROOM_TEMPERATURE
+
+
+# ---------------------------------------------------------------------------
+# Multigroup 2-group slab, k-eigenvalue.
+# sigma_t = [0.30, 1.20] /cm, sigma_s[from][to] = [[0.25, 0.03], [0, 1.05]],
+# nu sigma_f = [0.005, 0.18], chi = [1, 0].  The XML/descriptor scatter and
+# chi matrices are row = outgoing, column = incoming (Multigroup.cpp:124-147),
+# i.e. the transpose of sigma_s[from][to].
+C1_CAPTURE = [0.018, 0.078]
+C1_SCATTER = [0.25, 0.0, 0.03, 1.05]
+C1_FISSION = [0.002, 0.072]
+C1_NUBAR = [2.5, 2.5]
+C1_CHI = [1.0, 1.0, 0.0, 0.0]
+C1_KINF = 0.82  # (0.005 + 0.18 * 0.03 / 0.15) / 0.05
+
+
+def c1_nuclide():
+    return dict(capture=np.array(C1_CAPTURE), scatter=np.array(C1_SCATTER),
+                fission_xs=np.array(C1_FISSION), nubar=np.array(C1_NUBAR),
+                chi=np.array(C1_CHI))
+
+
+def c1_slab(width=20.0, histories=10000, flux_bins=100, half_space=None):
+    """1D slab [0, width] cm, vacuum on both sides (void cells outside).
+
+    half_space: if given, the slab is replaced by [-half_space, half_space]
+    (used as a quasi-infinite medium for the k_inf check)."""
+    pr = Problem(abi.MMC_MULTIGROUP, 2)
+    pr.add_field()
+    lo, hi = (0.0, width) if half_space is None else (-half_space, half_space)
+    s0 = pr.add_surface("left", abi.MMC_PLANEX, a=lo)
+    s1 = pr.add_surface("right", abi.MMC_PLANEX, a=hi)
+    pr.nuclides.append(c1_nuclide())
+    m = pr.add_material("fuel", 1.0, [(0, 1.0)])
+    pr.add_cell("slab", m, [(s0, False), (s1, True)])
+    pr.add_cell("", -1, [(s0, True)])
+    pr.add_cell("", -1, [(s1, False)])
+    pr.problem = "keigenvalue"
+    pr.inactive, pr.active = 50, 150
+    pr.histories = histories
+    pr.seed = 1
+    mid = 0.5 * (lo + hi)
+    pr.source = dict(position=(mid, 0.0, 0.0), dir_kind=abi.MMC_DIR_ISOTROPIC,
+                     E=1.0)
+    pr.estimators.append(dict(kind=abi.MMC_EST_FLUX_MESH, name="flux",
+                              mesh=dict(n=[flux_bins, 1, 1], lo=[lo, -1e30, -1e30],
+                                        hi=[hi, 1e30, 1e30])))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leak_left", surface=s0,
+                              energy=dict(kind=abi.MMC_BINS_LINSPACE, min=0.5,
+                                          max=2.5, bins=2)))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leak_right", surface=s1))
+    pr.entropy_mesh = ([100, 1, 1], [lo, -1e30, -1e30], [hi, 1e30, 1e30])
+    return pr
+
+
+def point_absorber_sphere(radius=1.0, sigma=1.0):
+    """The reference's point-source leakage problem (test/point_source_leakage.xml)."""
+    pr = Problem(abi.MMC_MULTIGROUP, 1)
+    pr.add_field()
+    s = pr.add_surface("sphere", abi.MMC_SPHERE, r=radius)
+    pr.nuclides.append(dict(capture=np.array([sigma])))
+    m = pr.add_material("absorber", 1.0, [(0, 1.0)])
+    pr.add_cell("sphere", m, [(s, True)])
+    pr.add_cell("", -1, [(s, False)])
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_ISOTROPIC, E=1.0)
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leakage", surface=s))
+    pr.histories = 1000
+    return pr

@@ -1,0 +1,217 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of the CPU restatement (oracle/_build/liboracle.so).  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module, and only as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from minimc_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(abi.WorldDesc), C.POINTER(abi.TallyDesc),
+                                 C.POINTER(abi.Options), C.POINTER(vp)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_destroy.restype = None
+        L.orc_last_error.argtypes = [vp]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_sample_source.argtypes = [vp, C.POINTER(abi.SourceDesc), C.c_uint64,
+                                        C.c_uint64, C.POINTER(abi.ParticleSoA)]
+        L.orc_eval_event.argtypes = [vp, C.c_int, C.POINTER(abi.ParticleSoA), C.c_uint64]
+        L.orc_run_fixed_source.argtypes = [vp, C.POINTER(abi.SourceDesc), C.c_uint64,
+                                           C.c_uint64, C.c_uint64, C.c_int]
+        L.orc_keig_init.argtypes = [vp, C.POINTER(abi.SourceDesc), C.c_uint64]
+        L.orc_run_cycle.argtypes = [vp, C.c_int, C.c_int, C.POINTER(abi.CycleStats)]
+        L.orc_get_source.argtypes = [vp, C.POINTER(abi.Site), C.c_uint64,
+                                     C.POINTER(C.c_uint64)]
+        L.orc_set_source.argtypes = [vp, C.POINTER(abi.Site), C.c_uint64]
+        L.orc_tally_size.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_get_tallies.argtypes = [vp, C.c_int, abi.c_double_p, abi.c_double_p,
+                                      C.c_uint64]
+        L.orc_reset_tallies.argtypes = [vp]
+        L.orc_get_counters.argtypes = [vp, C.POINTER(abi.Counters)]
+        L.orc_tsl_sample.argtypes = [vp, C.c_int, C.c_uint64, abi.c_double_p,
+                                     abi.c_double_p, abi.c_u32_p, abi.c_double_p,
+                                     abi.c_double_p, C.c_int]
+        L.orc_set_entropy_mesh.argtypes = [vp, C.POINTER(C.c_int32),
+                                           abi.c_double_p, abi.c_double_p]
+        L.orc_rng_draws.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.orc_rng_canonicals.argtypes = [C.c_uint64, C.c_uint64, abi.c_double_p]
+        L.orc_rng_exponentials.argtypes = [C.c_uint64, C.c_double, C.c_uint64,
+                                           abi.c_double_p]
+        L.orc_rng_directions.argtypes = [C.c_uint64, C.c_uint64, abi.c_double_p]
+        _lib = L
+    return _lib
+
+
+def rng_draws(seed, n):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_rng_draws(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def rng_canonicals(seed, n):
+    out = np.zeros(n)
+    lib().orc_rng_canonicals(seed, n, abi.dptr(out))
+    return out
+
+
+def rng_exponentials(seed, lam, n):
+    out = np.zeros(n)
+    lib().orc_rng_exponentials(seed, lam, n, abi.dptr(out))
+    return out
+
+
+def rng_directions(seed, n):
+    out = np.zeros((n, 3))
+    lib().orc_rng_directions(seed, n, abi.dptr(out))
+    return out
+
+
+class SoA:
+    """Host particle state in the mmc_particle_soa layout."""
+    DFIELDS = ("x", "y", "z", "u", "v", "w", "E", "out0", "out1")
+    IFIELDS = ("cell", "surface", "event", "iout")
+
+    def __init__(self, n, **kw):
+        self.n = n
+        for f in self.DFIELDS:
+            setattr(self, f, np.zeros(n))
+        for f in self.IFIELDS:
+            setattr(self, f, np.full(n, -1 if f in ("cell", "surface") else 0,
+                                     dtype=np.int32))
+        self.rng = np.ones(n, dtype=np.uint32)
+        for k, v in kw.items():
+            cur = getattr(self, k)
+            setattr(self, k, np.ascontiguousarray(np.broadcast_to(
+                np.asarray(v, dtype=cur.dtype), (n,))).copy())
+
+    def copy(self):
+        s = SoA(self.n)
+        for f in self.DFIELDS + self.IFIELDS + ("rng",):
+            setattr(s, f, getattr(self, f).copy())
+        return s
+
+    def struct(self):
+        s = abi.ParticleSoA()
+        for f in self.DFIELDS:
+            setattr(s, f, abi.dptr(getattr(self, f)))
+        for f in self.IFIELDS:
+            setattr(s, f, getattr(self, f).ctypes.data_as(abi.c_i32_p))
+        s.rng = self.rng.ctypes.data_as(abi.c_u32_p)
+        return s
+
+
+class Oracle:
+    """The CPU restatement behind the same runner interface as the product."""
+
+    def __init__(self, problem, tracking=None, max_steps=0):
+        self.problem = problem
+        self._w, self._t, self._keep = problem.descriptors()
+        opt = abi.Options()
+        opt.tracking = problem.tracking if tracking is None else tracking
+        opt.max_steps = max_steps
+        h = C.c_void_p()
+        rc = lib().orc_create(C.byref(self._w), C.byref(self._t), C.byref(opt),
+                              C.byref(h))
+        if rc:
+            raise RuntimeError("orc_create failed: %d" % rc)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError("oracle error %d: %s" % (
+                rc, lib().orc_last_error(self.h).decode()))
+
+    def sample_source(self, seed0, n, src=None):
+        out = SoA(n)
+        s = out.struct()
+        sd = self.problem.source_desc(src)
+        self._check(lib().orc_sample_source(self.h, C.byref(sd), seed0, n, C.byref(s)))
+        return out
+
+    def eval_event(self, kind, soa):
+        s = soa.struct()
+        self._check(lib().orc_eval_event(self.h, kind, C.byref(s), soa.n))
+        return soa
+
+    def run_fixed_source(self, n, seed0=None, first=0, threads=1, src=None):
+        sd = self.problem.source_desc(src)
+        seed0 = self.problem.seed if seed0 is None else seed0
+        self._check(lib().orc_run_fixed_source(self.h, C.byref(sd), seed0, first, n,
+                                               threads))
+
+    def set_entropy_mesh(self, n, lo, hi):
+        na = (C.c_int32 * 3)(*n)
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        self._check(lib().orc_set_entropy_mesh(self.h, na, abi.dptr(lo), abi.dptr(hi)))
+
+    def keig_init(self, n, src=None):
+        sd = self.problem.source_desc(src)
+        self._check(lib().orc_keig_init(self.h, C.byref(sd), n))
+
+    def run_cycle(self, active, threads=1):
+        st = abi.CycleStats()
+        self._check(lib().orc_run_cycle(self.h, 1 if active else 0, threads,
+                                        C.byref(st)))
+        return st.as_dict()
+
+    def get_source(self):
+        n = C.c_uint64()
+        self._check(lib().orc_get_source(self.h, None, 0, C.byref(n)))
+        arr = (abi.Site * n.value)()
+        self._check(lib().orc_get_source(self.h, arr, n.value, C.byref(n)))
+        return arr
+
+    def set_source(self, sites):
+        self._check(lib().orc_set_source(self.h, sites, len(sites)))
+
+    def tallies(self, e):
+        n = C.c_uint64()
+        self._check(lib().orc_tally_size(self.h, e, C.byref(n)))
+        S, S2 = np.zeros(n.value), np.zeros(n.value)
+        self._check(lib().orc_get_tallies(self.h, e, abi.dptr(S), abi.dptr(S2), n.value))
+        return S, S2
+
+    def reset_tallies(self):
+        self._check(lib().orc_reset_tallies(self.h))
+
+    def counters(self):
+        c = abi.Counters()
+        self._check(lib().orc_get_counters(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def tsl_sample(self, E, T, rng, tsl=0, threads=1):
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rng = np.ascontiguousarray(rng, dtype=np.uint32).copy()
+        a, b = np.zeros(len(E)), np.zeros(len(E))
+        self._check(lib().orc_tsl_sample(self.h, tsl, len(E), abi.dptr(E), abi.dptr(T),
+                                         rng.ctypes.data_as(abi.c_u32_p),
+                                         abi.dptr(a), abi.dptr(b), threads))
+        return a, b, rng
