@@ -312,10 +312,9 @@ enum mmc_event_kind {
                                    Nuclide::Interact; iout = number of
                                    secondaries banked                   */
   MMC_EVAL_SCATTER_ROTATE = 5,  /* Particle::Scatter(mu=out0, E'=out1)    */
-  MMC_EVAL_TSL_SAMPLE = 6,      /* ThermalScattering::Scatter pieces:
-                                   in E, T=out0; out0 = alpha, out1 = beta
-                                   (tsl 0 of the world)                  */
-  MMC_EVAL_SOURCE = 7           /* Source::Sample(seed = rng[i])          */
+  MMC_EVAL_TSL_SAMPLE = 6       /* ThermalScattering::SampleBeta then
+                                   SampleAlpha: in E, T = out0;
+                                   out0 = alpha, out1 = beta (tsl 0)     */
 };
 
 /* ---- counters ----------------------------------------------------------- */
@@ -397,6 +396,11 @@ int mmc_get_counters(mmc_ctx* ctx, mmc_counters* out);
 /* Single-event parity entry point: evaluates kind on n particle states
  * already in host memory (in place). */
 int mmc_eval_event(mmc_ctx* ctx, int kind, mmc_particle_soa* p, uint64_t n);
+
+/* Source::Sample (Source.cpp:143-154) for seeds seed0 + i, i < n, into host
+ * particle state (x..E, rng = the particle's seed, cell). */
+int mmc_sample_source(mmc_ctx* ctx, const mmc_source_desc* source,
+                      uint64_t seed0, uint64_t n, mmc_particle_soa* out);
 
 /* ---- thermal-scattering samplers on device-resident batches -------------
  * Pointers are device pointers.  E in MeV, T in K, rng minstd states

@@ -1,0 +1,637 @@
+// Device-side World tables and per-particle physics for gfx950.
+//
+// Every function here evaluates one reference function on one particle
+// state; the arithmetic keeps the reference's operation order and is
+// compiled with contraction off, so results equal the CPU's up to the
+// last-ulp differences of the libm calls (log, exp, sin, cos, erf, pow).
+// References are to the minimc sources (file:line).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mmc {
+
+// ---- constants (src/Constants.hpp:13-35) ---------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kNudge = 10 * 2.220446049250313080847e-16;
+constexpr double kOnAxis = 0.9;
+constexpr double kBoltzmann = 8.617333262145e-11;
+constexpr double kNeutronMass = 1.045354912280858e-18;
+constexpr double kRelTempTol = 0.01;
+constexpr int kBetaResampleLimit = 10;
+constexpr int kAlphaResampleLimit = 10;
+constexpr uint32_t kLcgM = 2147483647u;
+
+// error bits (device flag word)
+enum : uint32_t {
+  kErrLost = 1u,
+  kErrResample = 2u,
+  kErrSteps = 4u,
+  kErrAlphaRange = 8u,
+  kErrOverflow = 16u,
+  kErrReaction = 32u  // no reaction selected after kMaxReactionResamples
+};
+constexpr int kMaxReactionResamples = 1000;
+
+// ---- tables (flattened World) ---------------------------------------------
+struct DSurface {
+  double a, b, c, r;
+  int kind;
+  int pad;
+};
+struct DCell {
+  int material, face_begin, face_count, field;
+};
+struct DFace {
+  int surface, inside;
+};
+struct DField {
+  int kind, pad;
+  double c, gx, gy, gz, b, upper;
+};
+struct DMaterial {
+  double aden;
+  int comp_begin, comp_count;
+};
+struct DComp {
+  double afrac;
+  int nuclide, pad;
+};
+// A ContinuousMap on a (possibly shared) key grid.
+struct DTable {
+  int grid;   // index into grids
+  int y_off;  // values in pool
+};
+struct DGrid {
+  int x_off, n;         // keys in pool
+  int hash_off, n_hash; // bucket bounds in ipool (n_hash + 1 entries), 0 = none
+  double log_x0, inv_dlog;
+};
+struct DCeReaction {
+  int kind, tsl, xs, nubar;  // xs / nubar: table index (-1 none)
+  int chi_kind, pad;
+  double temperature, watt_a, watt_b;
+};
+struct DCeNuclide {
+  double awr, total_temperature;
+  int has_total, total;  // table
+  int rx_begin, rx_count;
+  int grid;              // the grid shared by every table of this nuclide, or -1
+  int fissile;
+};
+struct DTslPart {
+  int F_off, nF, cdf_off, S_off, x_off, nx, T_off, nT, xt_off, rank;
+};
+struct DTsl {
+  int majorant;          // table
+  int E_axis_off, nE, E_modes_off, T_axis_off, nT, T_modes_off, S_off, rank;
+  int Es_off, nEs, E_ends_off, n_bp, bp_begin;       // beta partitions
+  int betas_off, nbetas, beta_ends_off, n_ap, ap_begin;  // alpha partitions
+  int pad;
+  double beta_cutoff, alpha_cutoff, awr, cutoff_energy;
+};
+struct DBins {
+  int kind, n_bins;  // total bins including overflow bins
+  double lo, hi, base_log, width;
+  int bounds_off, n_bounds;
+};
+struct DEst {
+  int kind, surface, has_dir, pad;
+  double dx, dy, dz;
+  DBins cosine, energy;
+  int mesh_n[3], bin_off;  // bin_off: first global tally bin
+  double mesh_lo[3], mesh_hi[3];
+};
+
+struct DevWorld {
+  int energy_mode, G, tracking, n_cells;
+  int n_est, n_est_current, n_est_flux, n_est_tsl;
+  const DSurface* surfaces;
+  const DCell* cells;
+  const DFace* faces;
+  const DField* fields;
+  const DMaterial* mats;
+  const DComp* comps;
+  // multigroup: [nuc*G] and [nuc*G*G] ([nuc][g_in][g_out])
+  const double *mg_capture, *mg_scatter_xs, *mg_fission, *mg_nubar, *mg_total;
+  const double *mg_scatter_probs, *mg_chi_probs;
+  // continuous
+  const DCeNuclide* ce;
+  const DCeReaction* rx;
+  const DTable* tables;
+  const DGrid* grids;
+  const DTsl* tsl;
+  const DTslPart* parts;
+  const double* pool;
+  const int* ipool;
+  const DEst* ests;
+};
+
+// ---- RNG: std::minstd_rand with libstdc++ canonical (see oracle/rng.hpp) --
+__device__ __forceinline__ uint32_t LcgNext(uint32_t& x) {
+  const uint64_t p = uint64_t(x) * 48271u;
+  uint32_t r = uint32_t(p & 0x7fffffffu) + uint32_t(p >> 31);
+  if (r >= kLcgM) r -= kLcgM;
+  x = r;
+  return r;
+}
+__host__ __device__ __forceinline__ uint32_t LcgSeed(uint64_t s) {
+  uint32_t x = uint32_t(s % kLcgM);
+  return x == 0 ? 1u : x;
+}
+__device__ __forceinline__ double Canonical(uint32_t& x) {
+  const double r = 2147483646.0;
+  const double r2 = 2147483646.0 * 2147483646.0;
+  double sum = double(LcgNext(x) - 1u);
+  sum = sum + double(LcgNext(x) - 1u) * r;
+  double ret = sum / r2;
+  if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;  // nextafter(1, 0)
+  return ret;
+}
+__device__ __forceinline__ double Uniform(uint32_t& x, double a, double b) {
+  return Canonical(x) * (b - a) + a;
+}
+
+// ---- vectors (src/Point.cpp) ----------------------------------------------
+struct V3 {
+  double x, y, z;
+};
+__device__ __forceinline__ V3 Add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 Sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 Mul(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 Div(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double Dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 Cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 Normalized(V3 p) {
+  const double n = sqrt(Dot(p, p));
+  return {p.x / n, p.y / n, p.z / n};
+}
+// Direction(RNG&), Point.cpp:90-96
+__device__ __forceinline__ V3 IsotropicDirection(uint32_t& g) {
+  V3 d;
+  d.x = Uniform(g, -1., +1.);
+  const double sin_theta = sqrt(1 - d.x * d.x);
+  const double phi = Uniform(g, 0., 2 * kPi);
+  double s, c;
+  sincos(phi, &s, &c);
+  d.y = sin_theta * c;
+  d.z = sin_theta * s;
+  return d;
+}
+// Direction(d, mu, phi) then Normalize, Point.cpp:98-129
+__device__ __forceinline__ V3 Rotate(V3 d, double mu, double phi) {
+  const bool off = d.x <= kOnAxis && d.x > -kOnAxis;
+  const V3 ax = off ? V3{1, 0, 0} : V3{0, 1, 0};
+  const V3 u = Normalized(Cross(d, ax));
+  const V3 v = Normalized(Cross(d, u));
+  double s, c;
+  sincos(phi, &s, &c);
+  const double st = sqrt(1 - mu * mu);
+  const V3 uc = Mul(u, st * c);
+  const V3 vc = Mul(v, st * s);
+  const V3 dc = Mul(d, mu);
+  return Normalized(Add(Add(uc, vc), dc));
+}
+
+// ---- geometry (src/CSGSurface.cpp, src/Cell.cpp, src/World.cpp) ---------
+__device__ __forceinline__ double SolveQuadratic(double a, double b, double c) {
+  const double disc = b * b - 4 * a * c;
+  if (disc <= 0) return __builtin_huge_val();
+  const double sq = sqrt(disc);
+  const double lesser = b > 0 ? (-b - sq) / (2 * a) : (2 * c) / (-b + sq);
+  const double greater = b > 0 ? (2 * c) / (-b - sq) : (-b + sq) / (2 * a);
+  if (lesser > 0) return lesser;
+  if (greater > 0) return greater;
+  return __builtin_huge_val();
+}
+__device__ __forceinline__ double SurfDistance(const DSurface& s, V3 o, V3 d) {
+  switch (s.kind) {
+    case 0: {  // sphere
+      const V3 oc = Sub(o, V3{s.a, s.b, s.c});
+      return SolveQuadratic(1, 2 * Dot(oc, d), Dot(oc, oc) - s.r * s.r);
+    }
+    case 2: {  // cylinder along x
+      const double pp = Dot(o, o);
+      const double pw = Dot(o, V3{1, 0, 0});
+      const double dw = Dot(d, V3{1, 0, 0});
+      const double pd = Dot(o, d);
+      return SolveQuadratic(1 - dw * dw, 2 * (pd - pw * dw), pp - pw * pw - s.r * s.r);
+    }
+    default: {  // planes x (1), y (3), z (4)
+      const V3 ax = s.kind == 1 ? V3{1, 0, 0} : s.kind == 3 ? V3{0, 1, 0} : V3{0, 0, 1};
+      const double t = (s.a - Dot(o, ax)) / Dot(d, ax);
+      return t > 0 ? t : __builtin_huge_val();
+    }
+  }
+}
+__device__ __forceinline__ bool SurfContains(const DSurface& s, V3 p) {
+  switch (s.kind) {
+    case 0: {
+      const V3 pc = Sub(p, V3{s.a, s.b, s.c});
+      return Dot(pc, pc) < s.r * s.r;
+    }
+    case 2: {  // quirk Q1: sqrt(rho^2) < r^2 (CSGSurface.cpp:164-174)
+      const double pp = Dot(p, p);
+      const double pw = Dot(p, V3{1, 0, 0});
+      return sqrt(pp - pw * pw) < s.r * s.r;
+    }
+    case 1:
+      return Dot(p, V3{1, 0, 0}) < s.a;
+    case 3:
+      return Dot(p, V3{0, 1, 0}) < s.a;
+    default:
+      return Dot(p, V3{0, 0, 1}) < s.a;
+  }
+}
+__device__ __forceinline__ int FindCell(const DevWorld& W, V3 p) {
+  for (int c = 0; c < W.n_cells; c++) {
+    const DCell cell = W.cells[c];
+    bool in = true;
+    for (int k = 0; k < cell.face_count && in; k++) {
+      const DFace f = W.faces[cell.face_begin + k];
+      in = SurfContains(W.surfaces[f.surface], p) == (f.inside != 0);
+    }
+    if (in) return c;
+  }
+  return -1;
+}
+// Cell::NearestSurface (Cell.cpp:37-51): first minimum wins.
+__device__ __forceinline__ double NearestSurface(const DevWorld& W, int cell, V3 p,
+                                                 V3 d, int& surf) {
+  const DCell c = W.cells[cell];
+  double best = __builtin_huge_val();
+  surf = -1;
+  for (int k = 0; k < c.face_count; k++) {
+    const int s = W.faces[c.face_begin + k].surface;
+    const double t = SurfDistance(W.surfaces[s], p, d);
+    if (surf < 0 || t < best) {
+      best = t;
+      surf = s;
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ double FieldAt(const DField& f, V3 p) {
+  return f.kind == 0 ? f.c : Dot(V3{f.gx, f.gy, f.gz}, p) + f.b;
+}
+
+// ---- grid search and ContinuousMap::at (ContinuousMap.hpp:20-56) ---------
+// upper_bound(x[lo..hi), key) on ascending keys.
+__device__ __forceinline__ int UpperBoundRange(const double* x, int lo, int hi,
+                                               double key) {
+  int len = hi - lo;
+  while (len > 0) {
+    const int half = len >> 1;
+    const int mid = lo + half;
+    if (key < x[mid]) {
+      len = half;
+    } else {
+      lo = mid + 1;
+      len = len - half - 1;
+    }
+  }
+  return lo;
+}
+// The grid's log-spaced bucket bounds (staged in LDS by the caller when
+// `hash` points there) narrow the search to at most three buckets.
+__device__ __forceinline__ int GridUpperBound(const DevWorld& W, const DGrid& g,
+                                              const int* hash, double key) {
+  const double* x = W.pool + g.x_off;
+  int lo = 0, hi = g.n;
+  if (g.n_hash > 0 && key > x[0] && key < x[g.n - 1]) {
+    const double f = (log(key) - g.log_x0) * g.inv_dlog;
+    int b = int(f);
+    b = b < 0 ? 0 : (b >= g.n_hash ? g.n_hash - 1 : b);
+    const int b0 = b > 0 ? b - 1 : 0;
+    const int b1 = b + 2 < g.n_hash ? b + 2 : g.n_hash;
+    lo = hash[b0];
+    hi = hash[b1];
+  }
+  return UpperBoundRange(x, lo, hi, key);
+}
+__device__ __forceinline__ double TableAtIndex(const DevWorld& W, const DGrid& g,
+                                               int y_off, int hi, double k) {
+  const double* x = W.pool + g.x_off;
+  const double* y = W.pool + y_off;
+  if (hi == g.n) return y[hi - 1];
+  if (hi == 0) return y[0];
+  const double k_hi = x[hi], v_hi = y[hi], k_lo = x[hi - 1], v_lo = y[hi - 1];
+  return v_lo + (v_hi - v_lo) / (k_hi - k_lo) * (k - k_lo);
+}
+__device__ __forceinline__ double TableAt(const DevWorld& W, int t, double k) {
+  const DTable tb = W.tables[t];
+  const DGrid g = W.grids[tb.grid];
+  const int hi = GridUpperBound(W, g, W.ipool + g.hash_off, k);
+  return TableAtIndex(W, g, tb.y_off, hi, k);
+}
+// With a precomputed grid index (same grid).
+__device__ __forceinline__ double TableAtI(const DevWorld& W, int t, int hi, double k) {
+  const DTable tb = W.tables[t];
+  return TableAtIndex(W, W.grids[tb.grid], tb.y_off, hi, k);
+}
+
+// ---- thermal scattering POD (src/ThermalScattering.cpp) -------------------
+__device__ __forceinline__ int UpperBoundArr(const double* x, int n, double key) {
+  return UpperBoundRange(x, 0, n, key);
+}
+// BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256)
+__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const DTslPart& P,
+                                               int cdf_i, int x_i, double T) {
+  const double* Ts = W.pool + P.T_off;
+  const int cand = UpperBoundArr(Ts, P.nT, T);
+  const bool above = cand == P.nT;
+  const int t_hi = above ? cand - 1 : cand;
+  const bool below = t_hi == 0;
+  const int t_lo = below ? t_hi : t_hi - 1;
+  const double* S = W.pool + P.S_off;
+  const double* cdf = W.pool + P.cdf_off + cdf_i * P.rank;
+  const double* xh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
+  const double* xl = W.pool + P.xt_off + (x_i * P.nT + t_lo) * P.rank;
+  double v_hi = 0, v_lo = 0;
+  for (int o = 0; o < P.rank; o++) {
+    const double sc = S[o] * cdf[o];
+    v_hi = v_hi + sc * xh[o];
+    v_lo = v_lo + sc * xl[o];
+  }
+  const double T_hi = Ts[t_hi], T_lo = Ts[t_lo];
+  return v_lo + (v_hi - v_lo) / (T_hi - T_lo) * (T - T_lo);
+}
+// ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
+__device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, double E,
+                                           double T) {
+  const double* Ex = W.pool + t.E_axis_off;
+  const double* Tx = W.pool + t.T_axis_off;
+  const int e_hi = UpperBoundArr(Ex, t.nE, E);
+  const bool below_E = e_hi == 0;
+  const int e_lo = below_E ? e_hi : e_hi - 1;
+  const int cand = UpperBoundArr(Tx, t.nT, T);
+  const bool above_T = cand == t.nT;
+  const int t_hi = above_T ? cand - 1 : cand;
+  const bool below_T = t_hi == 0;
+  const int t_lo = below_T ? t_hi : t_hi - 1;
+  const double* S = W.pool + t.S_off;
+  const double* Em = W.pool + t.E_modes_off;
+  const double* Tm = W.pool + t.T_modes_off;
+  double ll = 0, lh = 0, hl = 0, hh = 0;
+  for (int o = 0; o < t.rank; o++) {
+    const double sel = S[o] * Em[e_lo * t.rank + o];
+    const double seh = S[o] * Em[e_hi * t.rank + o];
+    ll = ll + sel * Tm[t_lo * t.rank + o];
+    lh = lh + sel * Tm[t_hi * t.rank + o];
+    hl = hl + seh * Tm[t_lo * t.rank + o];
+    hh = hh + seh * Tm[t_hi * t.rank + o];
+  }
+  const double E_lo = Ex[e_lo], E_hi = Ex[e_hi];
+  const double r_E = below_E ? 1 : (E - E_lo) / (E_hi - E_lo);
+  const double xs_T_lo = ll + r_E * (hl - ll);
+  const double xs_T_hi = lh + r_E * (hh - lh);
+  const double T_lo = Tx[t_lo], T_hi = Tx[t_hi];
+  const double r_T = below_T ? 1 : above_T ? 0 : (T - T_lo) / (T_hi - T_lo);
+  return xs_T_lo + r_T * (xs_T_hi - xs_T_lo);
+}
+// ThermalScattering::SampleBeta (ThermalScattering.cpp:271-338)
+__device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uint32_t& g,
+                                           double E, double T, double& beta) {
+  const double* Es = W.pool + t.Es_off;
+  const int e_hi = UpperBoundArr(Es, t.nEs, E);
+  if (e_hi >= t.nEs) return false;  // E above the beta grid (assert at :278)
+  const double r = e_hi != 0 ? (E - Es[e_hi - 1]) / (Es[e_hi] - Es[e_hi - 1]) : 1;
+  const int e_s = r <= Canonical(g) ? e_hi - 1 : e_hi;
+  const double E_s = Es[e_s];
+  const int* ends = W.ipool + t.E_ends_off;
+  int p_i = 0;
+  while (p_i < t.n_bp && ends[p_i] <= e_s) p_i++;
+  const DTslPart P = W.parts[t.bp_begin + p_i];
+  const int local = p_i == 0 ? e_s : e_s - ends[p_i - 1];
+  const double* F = W.pool + P.F_off;
+  for (int k = 0; k < kBetaResampleLimit; k++) {
+    const double u = Canonical(g);
+    const int f_hi = UpperBoundArr(F, P.nF, u);
+    const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
+    const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
+    const double b_lo = f_hi != 0 ? PartEvaluate(W, P, f_hi - 1, local, T)
+                                  : -E_s / (kBoltzmann * T);
+    const double b_hi = f_hi != P.nF ? PartEvaluate(W, P, f_hi, local, T) : t.beta_cutoff;
+    const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
+    const double b_min = -E / (kBoltzmann * T);
+    if (b_min <= b) {
+      beta = b;
+      return true;
+    }
+  }
+  return false;
+}
+// ThermalScattering::SampleAlpha (ThermalScattering.cpp:340-463), quirk Q2
+// (operator precedence at :361-365) reproduced.
+__device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t,
+                                                uint32_t& g, double b, double E,
+                                                double T, double& alpha) {
+  const double* betas = W.pool + t.betas_off;
+  const double abs_b = fabs(b);
+  const int sgn = (0 < b) - (b < 0);
+  const int b_hi = UpperBoundArr(betas, t.nbetas, abs_b);
+  if (sgn != 0 && b_hi >= t.nbetas) return kErrAlphaRange;
+  const bool snap_lower = (sgn == 1 && t.beta_cutoff <= betas[b_hi]) ||
+                          (sgn == -1 && -betas[b_hi] < -E / (kBoltzmann * T));
+  const bool snap_min = b_hi == 0;
+  if (snap_lower && b_hi == 0) return kErrAlphaRange;
+  const double r = snap_lower ? 0
+                   : snap_min ? 1
+                              : (abs_b - betas[b_hi - 1] / (betas[b_hi] - betas[b_hi - 1]));
+  const int b_s_i = r <= Canonical(g) ? b_hi - 1 : b_hi;
+  const double b_s = sgn * betas[b_s_i];
+  const double sqrt_E = sqrt(E);
+  const double bs_root = sqrt(E + b_s * kBoltzmann * T);
+  const double akT = t.awr * kBoltzmann * T;
+  const double dm = sqrt_E - bs_root, dp = sqrt_E + bs_root;
+  const double a_min = dm * dm / akT;
+  const double a_max = dp * dp / akT;
+  const int* ends = W.ipool + t.beta_ends_off;
+  int p_i = 0;
+  while (p_i < t.n_ap && ends[p_i] <= b_s_i) p_i++;
+  const DTslPart P = W.parts[t.ap_begin + p_i];
+  const int local = p_i == 0 ? b_s_i : b_s_i - ends[p_i - 1];
+  const double* F = W.pool + P.F_off;
+  const int nF = P.nF;
+  double Fm[2];
+  const double targets[2] = {a_min, a_max};
+  for (int q = 0; q < 2; q++) {  // find_cdf (:395-418)
+    const double a = targets[q];
+    int first = 0, len = nF;
+    while (len > 0) {
+      const int half = len >> 1;
+      const int mid = first + half;
+      if (a < PartEvaluate(W, P, mid, local, T)) {
+        len = half;
+      } else {
+        first = mid + 1;
+        len = len - half - 1;
+      }
+    }
+    const int hi = first;
+    const double F_lo = hi != 0 ? F[hi - 1] : 0;
+    const double F_hi = hi != nF ? F[hi] : 1;
+    const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
+    const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
+    Fm[q] = F_lo + (a - a_lo) * (F_hi - F_lo) / (a_hi - a_lo);
+  }
+  for (int k = 0; k < kAlphaResampleLimit; k++) {
+    const double u = Fm[0] + Canonical(g) * (Fm[1] - Fm[0]);
+    const int hi = UpperBoundArr(F, nF, u);
+    const double F_lo = hi != 0 ? F[hi - 1] : 0;
+    const double F_hi = hi != nF ? F[hi] : 1;
+    const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
+    const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
+    const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
+    if (a_min < a && a < a_max) {
+      const double b_root = sqrt(E + b * kBoltzmann * T);
+      const double em = sqrt_E - b_root, ep = sqrt_E + b_root;
+      const double ba_min = em * em / akT;
+      const double ba_max = ep * ep / akT;
+      alpha = ba_min + (a - a_min) * (ba_max - ba_min) / (a_max - a_min);
+      return 0;
+    }
+  }
+  return kErrResample;
+}
+
+// ---- continuous-energy cross sections --------------------------------------
+__device__ __forceinline__ bool EvalValid(double T_eval, double T_req) {
+  return (T_req - T_eval) / T_eval < kRelTempTol;  // ContinuousEvaluation.cpp:17-20
+}
+__device__ __forceinline__ bool FreeGasValid(double awr, double E, double T) {
+  if (awr <= 1.0) return true;  // ContinuousReaction.cpp:207-223
+  return E < 500 * kBoltzmann * T / awr;
+}
+__device__ __forceinline__ double FreeGasAdjust(double awr, double E, double T) {
+  if (T == 0) return 1;  // ContinuousReaction.cpp:225-237
+  const double x = sqrt(E / (kBoltzmann * T));
+  const double arg = awr * x * x;
+  return (1 + 1 / (2 * arg)) * erf(sqrt(arg)) + exp(-arg) / sqrt(kPi * arg);
+}
+
+// Grid index of E for a nuclide whose tables share one grid, or -1.
+__device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNuclide& n,
+                                                const int* lds_hash, double E) {
+  if (n.grid < 0) return -1;
+  const DGrid g = W.grids[n.grid];
+  return GridUpperBound(W, g, lds_hash ? lds_hash : W.ipool + g.hash_off, E);
+}
+__device__ __forceinline__ double TabAt(const DevWorld& W, int t, int gi, double E) {
+  return gi >= 0 ? TableAtI(W, t, gi, E) : TableAt(W, t, E);
+}
+// ContinuousScatter::GetCrossSection / GetMajorant (ContinuousReaction.cpp:77-117)
+__device__ __forceinline__ double RxXs(const DevWorld& W, const DCeNuclide& n,
+                                       const DCeReaction& r, int gi, double E, double T,
+                                       bool majorant) {
+  if (r.kind != 1) return TabAt(W, r.xs, gi, E);
+  if (r.tsl >= 0) {
+    const DTsl& t = W.tsl[r.tsl];
+    if (E < t.cutoff_energy)
+      return majorant ? TableAt(W, t.majorant, E) : TslTotal(W, t, E, T);
+  }
+  if (EvalValid(r.temperature, T)) return TabAt(W, r.xs, gi, E);
+  if (FreeGasValid(n.awr, E, T))
+    return TabAt(W, r.xs, gi, E) / FreeGasAdjust(n.awr, E, r.temperature) *
+           FreeGasAdjust(n.awr, E, T);
+  return TabAt(W, r.xs, gi, E);
+}
+__device__ __forceinline__ bool ModifiesTotal(const DevWorld& W, const DCeNuclide& n,
+                                              double E) {
+  for (int k = 0; k < n.rx_count; k++) {
+    const DCeReaction r = W.rx[n.rx_begin + k];
+    if (r.kind == 1 && r.tsl >= 0 && E < W.tsl[r.tsl].cutoff_energy) return true;
+  }
+  return false;
+}
+// Continuous::GetTotal / GetMajorant (Continuous.cpp:26-55)
+__device__ __forceinline__ double CeTotal(const DevWorld& W, const DCeNuclide& n, int gi,
+                                          double E, double T, bool majorant) {
+  if (n.has_total && !ModifiesTotal(W, n, E) && EvalValid(n.total_temperature, T))
+    return TabAt(W, n.total, gi, E);
+  double acc = 0;
+  for (int k = 0; k < n.rx_count; k++)
+    acc = acc + RxXs(W, n, W.rx[n.rx_begin + k], gi, E, T, majorant);
+  return acc;
+}
+
+// Microscopic material total (or majorant): Material.cpp:41-61.
+__device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, double E,
+                                                double T, bool majorant,
+                                                const int* lds_hash) {
+  const DMaterial m = W.mats[mat];
+  double acc = 0;
+  if (W.energy_mode == 0) {
+    const int g = int(E) - 1;
+    for (int k = 0; k < m.comp_count; k++) {
+      const DComp c = W.comps[m.comp_begin + k];
+      acc = acc + c.afrac * W.mg_total[c.nuclide * W.G + g];
+    }
+    return acc;
+  }
+  for (int k = 0; k < m.comp_count; k++) {
+    const DComp c = W.comps[m.comp_begin + k];
+    const DCeNuclide n = W.ce[c.nuclide];
+    const int gi = NuclideGridIndex(W, n, lds_hash, E);
+    acc = acc + c.afrac * CeTotal(W, n, gi, E, T, majorant);
+  }
+  return acc;
+}
+// nu Sigma_f (microscopic, afrac-weighted) for the k estimators.
+__device__ __forceinline__ double MaterialNuFission(const DevWorld& W, int mat, double E) {
+  const DMaterial m = W.mats[mat];
+  double acc = 0;
+  for (int k = 0; k < m.comp_count; k++) {
+    const DComp c = W.comps[m.comp_begin + k];
+    if (W.energy_mode == 0) {
+      const int g = int(E) - 1;
+      acc += c.afrac * (W.mg_nubar[c.nuclide * W.G + g] * W.mg_fission[c.nuclide * W.G + g]);
+    } else {
+      const DCeNuclide n = W.ce[c.nuclide];
+      if (!n.fissile) continue;
+      double v = 0;
+      for (int q = 0; q < n.rx_count; q++) {
+        const DCeReaction r = W.rx[n.rx_begin + q];
+        if (r.kind == 2) v += TableAt(W, r.nubar, E) * TableAt(W, r.xs, E);
+      }
+      acc += c.afrac * v;
+    }
+  }
+  return acc;
+}
+
+// Watt spectrum sampling (additive extension; LA-9721 rule C64 Maxwellian).
+__device__ __forceinline__ double SampleWatt(uint32_t& g, double a, double b) {
+  const double xi1 = Canonical(g), xi2 = Canonical(g), xi3 = Canonical(g);
+  const double c = cos(kPi * xi3 / 2);
+  const double Em = -a * (log(xi1) + log(xi2) * c * c);
+  const double xi4 = Canonical(g);
+  return Em + a * a * b / 4 + (2 * xi4 - 1) * sqrt(a * a * b * Em);
+}
+
+// ---- tally bins (src/Bins.cpp) ---------------------------------------------
+__device__ __forceinline__ int BinIndex(const DevWorld& W, const DBins& b, double v) {
+  switch (b.kind) {
+    case 1:
+      if (v < b.lo) return 0;
+      if (v >= b.hi) return b.n_bins - 1;
+      return int((v - b.lo) / b.width + 1);
+    case 2: {
+      const double lv = log(v) / b.base_log;
+      if (lv < b.lo) return 0;
+      if (lv >= b.hi) return b.n_bins - 1;
+      return int((lv - b.lo) / b.width + 1);
+    }
+    case 3:
+      return UpperBoundArr(W.pool + b.bounds_off, b.n_bounds, v);
+    default:
+      return 0;
+  }
+}
+
+}  // namespace mmc

@@ -1,0 +1,679 @@
+// Event kernels of the tracker: source, advance (cross-section lookup +
+// distance to collision + CSG distance to surface + stream), surface
+// crossing, collision, thermal scattering; plus tally and bank kernels.
+//
+// Particle state lives in structure-of-arrays banks in HBM and stays in its
+// slot; each event kernel walks a queue of slot indices.  Queues are filled
+// with wave-wide ballots: every wave counts its pushes with one ballot and
+// reserves its range with one atomic.  Loops are block-uniform so all 64
+// lanes of a wave reach every ballot.
+#pragma once
+#include "mmc_device.hpp"
+#include "../../include/minimc_amd.h"
+
+namespace mmc {
+
+constexpr int kRing = 16;        // iteration counter ring
+constexpr int kRingStride = 8;   // counters per iteration
+enum RingSlot : int { kQ = 0, kC = 1, kX = 2, kT = 3 };
+// event counters (u64)
+enum Ev : int {
+  kEvAdvance = 0, kEvCollide, kEvCross, kEvVirtual, kEvTsl, kEvFallback,
+  kEvSites, kEvTail, kEvN
+};
+
+struct DevBank {
+  double *x, *y, *z, *u, *v, *w, *E;
+  uint32_t* rng;
+  int32_t* cell;
+  int32_t* surf;
+  uint32_t* hist;
+  uint32_t* steps;
+  double *k_col, *k_trk, *k_abs;
+  double* tslT;   // temperature of a deferred thermal-scattering collision
+  int32_t* tslI;  // its thermal-scattering table
+};
+
+struct RunState {
+  DevBank B;
+  uint32_t* Q[2];
+  uint32_t *QC, *QX, *QT;
+  uint32_t* ring;      // kRing * kRingStride
+  uint32_t* misc;      // [0] err bits, [1] records, [2] sites, [3] slots
+  unsigned long long* ev;
+  uint32_t capacity;   // slots
+  int keig;
+  uint32_t max_steps;
+  uint32_t chunk_first;  // first history of the chunk (fixed source)
+  // tallies
+  uint32_t* rec_hist;
+  uint32_t* rec_bin;
+  double* rec_val;
+  uint32_t rec_cap;
+  double* cycS;        // k-eig cycle sums per global bin
+  // fission sites (k-eig)
+  mmc_site* sites;
+  uint32_t site_cap;
+  uint32_t* fis_cnt;   // children per history (k-eig)
+  int tsl_surrogate;
+  float* mlp_in;       // [cap * 4] surrogate inputs of the TSL queue
+  float* mlp_out;      // [cap * 2]
+};
+
+__device__ __forceinline__ uint32_t WavePush(uint32_t* counter, bool pred,
+                                             uint32_t& total_out) {
+  const uint64_t mask = __ballot(pred);
+  const uint32_t total = __popcll(mask);
+  total_out = total;
+  if (total == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(counter, total);
+  base = __shfl(base, 0);
+  return base + prefix;
+}
+__device__ __forceinline__ void WaveCount(unsigned long long* c, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  if (__lane_id() == 0 && mask) atomicAdd(c, (unsigned long long)__popcll(mask));
+}
+__device__ __forceinline__ uint32_t* Ring(const RunState& R, int iter, int slot) {
+  return R.ring + ((iter + kRing) % kRing) * kRingStride + slot;
+}
+
+__device__ __forceinline__ V3 LoadPos(const DevBank& B, uint32_t p) {
+  return {B.x[p], B.y[p], B.z[p]};
+}
+__device__ __forceinline__ V3 LoadDir(const DevBank& B, uint32_t p) {
+  return {B.u[p], B.v[p], B.w[p]};
+}
+
+// Score one value into a global tally bin: per-history records for fixed
+// source, cycle sums for k-eigenvalue.
+__device__ __forceinline__ void ScoreBin(const RunState& R, uint32_t hist, int bin,
+                                         double val) {
+  if (R.keig) {
+    unsafeAtomicAdd(R.cycS + bin, val);
+    return;
+  }
+  const uint32_t k = atomicAdd(R.misc + 1, 1u);
+  if (k >= R.rec_cap) {
+    atomicOr(R.misc, kErrOverflow);
+    return;
+  }
+  R.rec_hist[k] = hist - R.chunk_first;
+  R.rec_bin[k] = uint32_t(bin);
+  R.rec_val[k] = val;
+}
+
+// CurrentEstimator::GetScore + ParticleBins::GetIndex
+// (Estimator.cpp:142-151, Bins.cpp:196-204)
+__device__ __forceinline__ void ScoreCurrent(const DevWorld& W, const RunState& R,
+                                             uint32_t hist, int surf, V3 d, double E) {
+  for (int e = 0; e < W.n_est; e++) {
+    const DEst& est = W.ests[e];
+    if (est.kind != 0 || est.surface != surf) continue;
+    const int ci = est.has_dir ? BinIndex(W, est.cosine, Dot(V3{est.dx, est.dy, est.dz}, d)) : 0;
+    const int ei = BinIndex(W, est.energy, E);
+    ScoreBin(R, hist, est.bin_off + est.energy.n_bins * ci + ei, 1.0);
+  }
+}
+__device__ __forceinline__ void ScoreFlux(const DevWorld& W, const RunState& R,
+                                          uint32_t hist, V3 p, double sigma_macro) {
+  for (int e = 0; e < W.n_est; e++) {
+    const DEst& est = W.ests[e];
+    if (est.kind != 1) continue;
+    const double q[3] = {p.x, p.y, p.z};
+    int idx = 0;
+    bool in = true;
+    for (int a = 0; a < 3; a++) {
+      const double f = (q[a] - est.mesh_lo[a]) / (est.mesh_hi[a] - est.mesh_lo[a]);
+      if (!(f >= 0 && f < 1)) {
+        in = false;
+        break;
+      }
+      int i = int(f * est.mesh_n[a]);
+      if (i >= est.mesh_n[a]) i = est.mesh_n[a] - 1;
+      idx = idx * est.mesh_n[a] + i;
+    }
+    if (in) ScoreBin(R, hist, est.bin_off + idx, 1.0 / sigma_macro);
+  }
+}
+__device__ __forceinline__ void ScoreAlphaBeta(const DevWorld& W, const RunState& R,
+                                               uint32_t hist, double a, double b) {
+  for (int e = 0; e < W.n_est; e++) {
+    const DEst& est = W.ests[e];
+    if (est.kind == 2) ScoreBin(R, hist, est.bin_off + BinIndex(W, est.energy, a), 1.0);
+    if (est.kind == 3) ScoreBin(R, hist, est.bin_off + BinIndex(W, est.energy, b), 1.0);
+  }
+}
+
+#ifdef MMC_TRACKER_KERNELS
+// ---- source ---------------------------------------------------------------
+// Source::Sample (Source.cpp:143-154) for history first + i, seed seed0 + hist.
+__global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t seed0,
+                             uint32_t first, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) R.misc[3] = n;
+  const bool valid = i < n;
+  bool ok = false;
+  if (valid) {
+  const uint32_t hist = first + i;
+  uint32_t g = LcgSeed(seed0 + hist);
+  double q[3] = {s.x, s.y, s.z};
+  for (int a = 0; a < 3; a++)
+    if (s.box_hi[a] > s.box_lo[a]) q[a] = Uniform(g, s.box_lo[a], s.box_hi[a]);
+  V3 d;
+  if (s.dir_kind == 1) {
+    d = IsotropicDirection(g);
+  } else if (s.dir_kind == 2) {
+    const double mu = sqrt(Canonical(g));
+    const double phi = Uniform(g, 0., 2 * kPi);
+    d = Rotate(Normalized(V3{s.u, s.v, s.w}), mu, phi);
+  } else {
+    d = Normalized(V3{s.u, s.v, s.w});
+  }
+  const double E = s.energy_kind == 1 ? Uniform(g, s.E, s.E_hi) : s.E;
+  const uint32_t seed = LcgSeed(LcgNext(g));
+  const V3 p{q[0], q[1], q[2]};
+  const int cell = FindCell(W, p);
+  if (cell < 0) atomicOr(R.misc, kErrLost);
+  R.B.x[i] = p.x; R.B.y[i] = p.y; R.B.z[i] = p.z;
+  R.B.u[i] = d.x; R.B.v[i] = d.y; R.B.w[i] = d.z;
+  R.B.E[i] = E;
+  R.B.rng[i] = seed;
+  R.B.cell[i] = cell;
+  R.B.surf[i] = -1;
+  R.B.hist[i] = hist;
+  R.B.steps[i] = 0;
+  ok = cell >= 0;
+  }
+  uint32_t tot;
+  const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
+  if (ok) R.Q[0][iq] = i;
+}
+
+// k-eigenvalue: source site j -> particle slot j.
+__global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) R.misc[3] = n;
+  const bool valid = i < n;
+  bool ok = false;
+  if (valid) {
+  const mmc_site s = src[i];
+  const V3 p{s.x, s.y, s.z};
+  const int cell = FindCell(W, p);
+  if (cell < 0) atomicOr(R.misc, kErrLost);
+  R.B.x[i] = s.x; R.B.y[i] = s.y; R.B.z[i] = s.z;
+  R.B.u[i] = s.u; R.B.v[i] = s.v; R.B.w[i] = s.w;
+  R.B.E[i] = s.E;
+  R.B.rng[i] = LcgSeed(s.seed);
+  R.B.cell[i] = cell;
+  R.B.surf[i] = -1;
+  R.B.hist[i] = i;
+  R.B.steps[i] = 0;
+  R.B.k_col[i] = 0;
+  R.B.k_trk[i] = 0;
+  R.B.k_abs[i] = 0;
+  R.fis_cnt[i] = 0;
+  ok = cell >= 0;
+  }
+  uint32_t tot;
+  const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
+  if (ok) R.Q[0][iq] = i;
+}
+
+// ---- advance: XS lookup + distance to collision + CSG distance ------------
+// SurfaceTracking/CellDeltaTracking::Transport loop body up to the branch
+// (TransportMethod.cpp:57-66, :93-118).
+__global__ void __launch_bounds__(256) KAdvance(DevWorld W, RunState R, int iter, int qin) {
+  extern __shared__ int lds_hash[];
+  const int* hash = nullptr;
+  if (W.energy_mode == 1 && W.grids[0].n_hash > 0) {
+    const DGrid g0 = W.grids[0];
+    for (int k = threadIdx.x; k <= g0.n_hash; k += blockDim.x)
+      lds_hash[k] = W.ipool[g0.hash_off + k];
+    __syncthreads();
+    hash = lds_hash;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
+  const uint32_t n = *Ring(R, iter - 1, kQ);
+  const uint32_t* Q = R.Q[qin];
+  const bool delta = W.tracking == 1;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    bool to_x = false, to_c = false, to_q = false;
+    uint32_t p = 0;
+    if (valid) {
+      p = Q[q];
+      V3 pos = LoadPos(R.B, p);
+      const V3 dir = LoadDir(R.B, p);
+      const double E = R.B.E[p];
+      uint32_t g = R.B.rng[p];
+      const int cell = R.B.cell[p];
+      const DCell c = W.cells[cell];
+      const DField fld = W.fields[c.field];
+      const double aden = W.mats[c.material].aden;
+      const double T = delta ? (fld.kind == 0 ? fld.c : fld.upper) : FieldAt(fld, pos);
+      const double sigma = MaterialTotal(W, c.material, E, T, delta, hash);
+      const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
+      int s;
+      const double d_surf = NearestSurface(W, cell, pos, dir, s);
+      const bool cross = delta ? (d_surf < d_coll) : !(d_coll < d_surf);
+      double d;
+      if (cross) {
+        d = d_surf + kNudge;
+        R.B.surf[p] = s;
+        to_x = true;
+      } else {
+        bool real = true;
+        if (delta) {
+          const double tot =
+              MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash);
+          real = Canonical(g) < tot / sigma;  // pre-flight position (quirk Q6)
+        }
+        d = d_coll;
+        to_c = real;
+        to_q = !real;
+      }
+      if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E);
+      pos = Add(pos, Mul(dir, d));
+      R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
+      R.B.rng[p] = g;
+      const uint32_t st = R.B.steps[p] + 1;
+      R.B.steps[p] = st;
+      if (st > R.max_steps) {
+        atomicOr(R.misc, kErrSteps);
+        to_x = to_c = to_q = false;
+      }
+    }
+    uint32_t tot;
+    const uint32_t ix = WavePush(Ring(R, iter, kX), to_x, tot);
+    if (to_x) R.QX[ix] = p;
+    const uint32_t ic = WavePush(Ring(R, iter, kC), to_c, tot);
+    if (to_c) R.QC[ic] = p;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), to_q, tot);
+    if (to_q) R.Q[qin ^ 1][iq] = p;
+    WaveCount(R.ev + kEvAdvance, valid);
+    WaveCount(R.ev + kEvVirtual, to_q);
+  }
+}
+
+// ---- surface crossing (TransportMethod.cpp:67-73, :98-105) ----------------
+__global__ void __launch_bounds__(256) KCross(DevWorld W, RunState R, int iter, int qin) {
+  const uint32_t n = *Ring(R, iter, kX);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    bool alive = false;
+    uint32_t p = 0;
+    if (valid) {
+      p = R.QX[q];
+      const V3 pos = LoadPos(R.B, p);
+      const int cell = FindCell(W, pos);
+      if (cell < 0) {
+        atomicOr(R.misc, kErrLost);
+      } else {
+        R.B.cell[p] = cell;
+        alive = W.cells[cell].material >= 0;  // surface_cross vs leak
+        if (W.n_est_current)
+          ScoreCurrent(W, R, R.B.hist[p], R.B.surf[p], LoadDir(R.B, p), R.B.E[p]);
+      }
+    }
+    uint32_t tot;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
+    if (alive) R.Q[qin ^ 1][iq] = p;
+    WaveCount(R.ev + kEvCross, valid);
+  }
+}
+
+#endif  // MMC_TRACKER_KERNELS
+
+// ---- collisions ------------------------------------------------------------
+__device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, uint32_t p,
+                                          uint32_t& g, V3 pos, V3 d, double E, int child,
+                                          uint32_t* live_q, uint32_t* live_cnt) {
+  const uint32_t seed = LcgNext(g);  // Particle::BankSecondaries (Particle.cpp:96-100)
+  const uint32_t hist = R.B.hist[p];
+  if (R.keig) {
+    const uint32_t k = atomicAdd(R.misc + 2, 1u);
+    if (k >= R.site_cap) {
+      atomicOr(R.misc, kErrOverflow);
+      return;
+    }
+    mmc_site s;
+    s.x = pos.x; s.y = pos.y; s.z = pos.z;
+    s.u = d.x; s.v = d.y; s.w = d.z;
+    s.E = E;
+    s.seed = seed;
+    s.pad = 0;
+    s.key = uint64_t(hist) * 64u + uint64_t(child);
+    R.sites[k] = s;
+    return;
+  }
+  // fixed source: the child is a new particle of the same history, queued
+  // with the survivors of this iteration (FixedSource.cpp:63-72)
+  const uint32_t slot = atomicAdd(R.misc + 3, 1u);
+  if (slot >= R.capacity) {
+    atomicOr(R.misc, kErrOverflow);
+    return;
+  }
+  R.B.x[slot] = pos.x; R.B.y[slot] = pos.y; R.B.z[slot] = pos.z;
+  R.B.u[slot] = d.x; R.B.v[slot] = d.y; R.B.w[slot] = d.z;
+  R.B.E[slot] = E;
+  R.B.rng[slot] = LcgSeed(seed);
+  R.B.cell[slot] = R.B.cell[p];
+  R.B.surf[slot] = -1;
+  R.B.hist[slot] = hist;
+  R.B.steps[slot] = 0;
+  if (live_q) live_q[atomicAdd(live_cnt, 1u)] = slot;
+}
+
+// Free-gas elastic scatter (ContinuousReaction.cpp:119-189).
+__device__ __forceinline__ void FreeGasScatter(double awr, double T, uint32_t& g, V3& dir,
+                                               double& E_io) {
+  const double m_n = kNeutronMass;
+  const double E = E_io;
+  const double s_n = sqrt(2. * E / m_n);
+  const V3 v_n = Mul(dir, s_n);
+  const double beta = sqrt((awr * m_n) / (2. * kBoltzmann * T));
+  const double y = beta * s_n;
+  double x, mu;
+  for (;;) {
+    const double xi_1 = Canonical(g), xi_2 = Canonical(g);
+    if (Canonical(g) < 2 / (sqrt(kPi) * y + 2)) {
+      x = sqrt(-log(xi_1 * xi_2));
+    } else {
+      const double xi_3 = Canonical(g);
+      const double z = cos(kPi * xi_3 / 2);
+      x = sqrt(-log(xi_1) - log(xi_2) * z * z);
+    }
+    mu = 2 * Canonical(g) - 1;
+    if (!(Canonical(g) >= sqrt(x * x + y * y - 2 * x * y * mu) / (x + y))) break;
+  }
+  const double s_T = x / beta;
+  const double phi = 2 * kPi * Canonical(g);
+  const V3 v_T = Mul(Rotate(dir, mu, phi), s_T);
+  const V3 v_cm = Div(Add(v_n, Mul(v_T, awr)), (1 + awr));
+  const V3 V_n = Sub(v_n, v_cm);
+  const double mu_cm = 2 * Canonical(g) - 1;
+  const double phi_cm = 2 * kPi * Canonical(g);
+  const V3 Vp_dir = Rotate(Normalized(V_n), mu_cm, phi_cm);
+  const V3 V_n_prime = Mul(Vp_dir, sqrt(Dot(V_n, V_n)));
+  const V3 v_n_prime = Add(V_n_prime, v_cm);
+  const double E_prime = 0.5 * m_n * Dot(v_n_prime, v_n_prime);
+  const double mu_lab = Dot(dir, Normalized(v_n_prime));
+  // Particle::Scatter (Particle.cpp:55-64)
+  const double phi2 = 2 * kPi * Canonical(g);
+  dir = Rotate(dir, mu_lab, phi2);
+  E_io = E_prime;
+}
+
+// Outcome of one collision on one particle (shared by the tracker and the
+// single-event parity kernel).
+struct CollideOut {
+  int event;      // mmc_event
+  int children;
+  bool tsl;       // deferred thermal scatter (tracker) / done (eval)
+};
+
+// Particle::SampleNuclide + Nuclide::Interact (Particle.cpp:110-124,
+// Multigroup.cpp:49-73,249-287, Continuous.cpp:57-70,
+// ContinuousReaction.cpp:60-62,119-189,252-265).  A thermal-scattering
+// scatter stops after choosing the reaction (out.tsl); the caller samples
+// (alpha, beta).
+__device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, V3 pos,
+                              V3& dir, double& E, uint32_t& g, int cell, uint32_t* live_q,
+                              uint32_t* live_cnt, const int* hash, double* k_abs,
+                              int* tsl_index) {
+  CollideOut out{MMC_EV_SCATTER, 0, false};
+  const DCell c = W.cells[cell];
+  const DMaterial m = W.mats[c.material];
+  const double T = FieldAt(W.fields[c.field], pos);
+  // SampleNuclide
+  const double threshold_n = MaterialTotal(W, c.material, E, T, false, hash) * Canonical(g);
+  int nuc = W.comps[m.comp_begin + m.comp_count - 1].nuclide;
+  {
+    double acc = 0;
+    for (int k = 0; k < m.comp_count; k++) {
+      const DComp cp = W.comps[m.comp_begin + k];
+      double tn;
+      if (W.energy_mode == 0) {
+        tn = W.mg_total[cp.nuclide * W.G + int(E) - 1];
+      } else {
+        const DCeNuclide n = W.ce[cp.nuclide];
+        tn = CeTotal(W, n, NuclideGridIndex(W, n, hash, E), E, T, false);
+      }
+      acc += cp.afrac * tn;
+      if (acc > threshold_n) {
+        nuc = cp.nuclide;
+        break;
+      }
+    }
+  }
+  if (W.energy_mode == 0) {
+    const int G = W.G;
+    const int gi = int(E) - 1;
+    const double cap = W.mg_capture[nuc * G + gi];
+    const double sc = W.mg_scatter_xs[nuc * G + gi];
+    const double fi = W.mg_fission[nuc * G + gi];
+    const double threshold = Canonical(g) * W.mg_total[nuc * G + gi];
+    double acc = 0;
+    acc += cap;
+    if (acc > threshold) {
+      out.event = MMC_EV_CAPTURE;
+    } else {
+      acc += sc;
+      if (acc > threshold) {
+        out.event = MMC_EV_SCATTER;
+        const double t = Canonical(g);
+        double a = 0;
+        const double* probs = W.mg_scatter_probs + (size_t(nuc) * G + gi) * G;
+        for (int go = 0; go < G; go++) {
+          a += probs[go];
+          if (a > t) {
+            E = double(go + 1);
+            dir = IsotropicDirection(g);
+            break;
+          }
+        }
+      } else {
+        acc += fi;
+        if (acc > threshold) {
+          out.event = MMC_EV_FISSION;
+          const int yield = int(W.mg_nubar[nuc * G + gi] + Canonical(g));
+          const double* probs = W.mg_chi_probs + (size_t(nuc) * G + gi) * G;
+          for (int i = 0; i < yield; i++) {
+            const double t = Canonical(g);
+            double a = 0;
+            for (int go = 0; go < G; go++) {
+              a += probs[go];
+              if (a > t) {
+                const V3 d = IsotropicDirection(g);
+                if (R) BankChild(W, *R, p, g, pos, d, double(go + 1), out.children, live_q, live_cnt);
+                else LcgNext(g);
+                out.children++;
+                break;
+              }
+            }
+          }
+        } else {
+          out.event = -1;  // no reaction selected (Multigroup.cpp:70-72): unchanged
+        }
+      }
+    }
+    if (k_abs && (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) &&
+        cap + fi > 0)
+      *k_abs += (W.mg_nubar[nuc * G + gi] * fi) / (cap + fi);
+    return out;
+  }
+  const DCeNuclide n = W.ce[nuc];
+  const int gi = NuclideGridIndex(W, n, hash, E);
+  for (int attempt = 0;; attempt++) {  // Continuous::Interact; tail recursion (Q4)
+    if (attempt >= kMaxReactionResamples) {
+      out.event = MMC_EV_CAPTURE;
+      out.children = -1;  // flags kErrReaction to the caller
+      return out;
+    }
+    const double threshold = Canonical(g) * CeTotal(W, n, gi, E, T, false);
+    double acc = 0;
+    for (int k = 0; k < n.rx_count; k++) {
+      const DCeReaction r = W.rx[n.rx_begin + k];
+      acc += RxXs(W, n, r, gi, E, T, false);
+      if (!(acc > threshold)) continue;
+      if (r.kind == 0) {
+        out.event = MMC_EV_CAPTURE;
+      } else if (r.kind == 1) {
+        out.event = MMC_EV_SCATTER;
+        if (r.tsl >= 0 && E < W.tsl[r.tsl].cutoff_energy) {
+          out.tsl = true;
+          *tsl_index = r.tsl;
+          return out;  // the caller samples (alpha, beta)
+        }
+        double Eo = E;
+        FreeGasScatter(n.awr, T, g, dir, Eo);
+        E = Eo;
+      } else {
+        out.event = MMC_EV_FISSION;
+        const int yield = int(TabAt(W, r.nubar, -1, E) + Canonical(g));
+        for (int i = 0; i < yield; i++) {
+          const V3 d = IsotropicDirection(g);
+          const double Ec = r.chi_kind == 1 ? SampleWatt(g, r.watt_a, r.watt_b) : E;
+          if (R) BankChild(W, *R, p, g, pos, d, Ec, out.children, live_q, live_cnt);
+          else LcgNext(g);
+          out.children++;
+        }
+      }
+      if (k_abs && (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) &&
+          n.fissile) {
+        double sa = 0, nusf = 0;
+        for (int q = 0; q < n.rx_count; q++) {
+          const DCeReaction rr = W.rx[n.rx_begin + q];
+          if (rr.kind == 0) sa += TabAt(W, rr.xs, gi, E);
+          if (rr.kind == 2) {
+            const double sf = TabAt(W, rr.xs, gi, E);
+            sa += sf;
+            nusf += TableAt(W, rr.nubar, E) * sf;
+          }
+        }
+        if (sa > 0) *k_abs += nusf / sa;
+      }
+      return out;
+    }
+  }
+}
+
+#ifdef MMC_TRACKER_KERNELS
+__global__ void __launch_bounds__(256) KCollide(DevWorld W, RunState R, int iter, int qin) {
+  extern __shared__ int lds_hash[];
+  const int* hash = nullptr;
+  if (W.energy_mode == 1 && W.grids[0].n_hash > 0) {
+    const DGrid g0 = W.grids[0];
+    for (int k = threadIdx.x; k <= g0.n_hash; k += blockDim.x)
+      lds_hash[k] = W.ipool[g0.hash_off + k];
+    __syncthreads();
+    hash = lds_hash;
+  }
+  const uint32_t n = *Ring(R, iter, kC);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    bool alive = false, to_t = false;
+    uint32_t p = 0;
+    if (valid) {
+      p = R.QC[q];
+      const V3 pos = LoadPos(R.B, p);
+      V3 dir = LoadDir(R.B, p);
+      double E = R.B.E[p];
+      uint32_t g = R.B.rng[p];
+      const int cell = R.B.cell[p];
+      const DCell c = W.cells[cell];
+      const double T = FieldAt(W.fields[c.field], pos);
+      if (W.n_est_flux || R.keig) {
+        const double tot = MaterialTotal(W, c.material, E, T, false, hash);
+        if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
+        if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E) / tot;
+      }
+      double kabs = 0;
+      int tsl = -1;
+      const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, R.Q[qin ^ 1],
+                                   Ring(R, iter, kQ), hash, R.keig ? &kabs : nullptr, &tsl);
+      if (o.children < 0) atomicOr(R.misc, kErrReaction);
+      if (R.keig) {
+        R.B.k_abs[p] += kabs;
+        if (o.event == MMC_EV_FISSION) R.fis_cnt[R.B.hist[p]] = uint32_t(o.children);
+      }
+      if (o.tsl) {
+        to_t = true;
+        R.B.tslT[p] = T;
+        R.B.tslI[p] = tsl;
+      } else {
+        alive = o.event == MMC_EV_SCATTER || o.event < 0;
+      }
+      R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+      R.B.E[p] = E;
+      R.B.rng[p] = g;
+    }
+    uint32_t tot;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
+    if (alive) R.Q[qin ^ 1][iq] = p;
+    const uint32_t it = WavePush(Ring(R, iter, kT), to_t, tot);
+    if (to_t) R.QT[it] = p;
+    WaveCount(R.ev + kEvCollide, valid);
+  }
+}
+
+#endif  // MMC_TRACKER_KERNELS
+
+// ThermalScattering::Scatter (ThermalScattering.cpp:159-171) with POD sampling.
+__device__ __forceinline__ uint32_t TslScatterPod(const DevWorld& W, const DTsl& t,
+                                                  uint32_t& g, double T, V3& dir,
+                                                  double& E, double& alpha, double& beta) {
+  const double E0 = E;
+  if (!SampleBeta(W, t, g, E0, T, beta)) return kErrResample;
+  const uint32_t e = SampleAlpha(W, t, g, beta, E0, T, alpha);
+  if (e) return e;
+  const double E_p = E0 + beta * kBoltzmann * T;
+  const double mu = (E0 + E_p - alpha * t.awr * kBoltzmann * T) / (2 * sqrt(E0 * E_p));
+  const double phi = 2 * kPi * Canonical(g);
+  dir = Rotate(dir, mu, phi);
+  E = E_p;
+  return 0;
+}
+
+#ifdef MMC_TRACKER_KERNELS
+__global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
+  const uint32_t n = *Ring(R, iter, kT);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    bool alive = false;
+    uint32_t p = 0;
+    if (valid) {
+      p = R.QT[q];
+      V3 dir = LoadDir(R.B, p);
+      double E = R.B.E[p];
+      uint32_t g = R.B.rng[p];
+      double a = 0, b = 0;
+      const uint32_t e = TslScatterPod(W, W.tsl[R.B.tslI[p]], g, R.B.tslT[p], dir, E, a, b);
+      if (e) {
+        atomicOr(R.misc, e);
+      } else {
+        alive = true;
+        if (W.n_est_tsl) ScoreAlphaBeta(W, R, R.B.hist[p], a, b);
+        R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+        R.B.E[p] = E;
+        R.B.rng[p] = g;
+      }
+    }
+    uint32_t tot;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
+    if (alive) R.Q[qin ^ 1][iq] = p;
+    WaveCount(R.ev + kEvTsl, valid);
+  }
+}
+
+#endif  // MMC_TRACKER_KERNELS
+
+}  // namespace mmc

@@ -1,0 +1,41 @@
+// Thermal-scattering surrogate: state shared between the tracker and the
+// MLP kernels (mmc_mlp.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mmc_kernels.hpp"
+
+namespace mmc {
+
+constexpr int kMlpMaxLayers = 8;
+
+struct MlpState {
+  bool ready = false;
+  int tsl = 0;
+  int n_layers = 0;
+  int widths[kMlpMaxLayers + 1] = {};
+  int w_off[kMlpMaxLayers] = {}, b_off[kMlpMaxLayers] = {};
+  long n_params = 0;
+  float in_shift[4] = {}, in_scale[4] = {}, out_shift[2] = {}, out_scale[2] = {};
+  double e_max = 0, t_lo = 0, t_hi = 0;
+  float* params = nullptr;        // FP32 master parameters
+  unsigned short* wbf16 = nullptr;  // BF16 weights in the inference layout
+  float *adam_m = nullptr, *adam_v = nullptr, *grads = nullptr;
+  float* act = nullptr;           // training activations
+  size_t act_cap = 0;
+  float* loss = nullptr;
+  long adam_t = 0;
+  float* xin = nullptr;           // tracker staging [cap x 4]
+  float* yout = nullptr;          // [cap x 2]
+  uint32_t* fb = nullptr;         // fallback queue
+  uint32_t stage_cap = 0;
+};
+
+void MlpFree(MlpState& m);
+// One thermal-scattering step of the tracker with the surrogate: gathers
+// (log E, T, u1, u2) of the TSL queue, runs the network, applies the
+// scatter, and sends out-of-box inputs to the POD sampler.
+int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
+               uint32_t grid, hipStream_t stream);
+
+}  // namespace mmc

@@ -1,0 +1,1310 @@
+// C ABI of the tracker (include/minimc_amd.h): context creation (World
+// flattening into device tables), the event loop driver, per-history tally
+// commit, the k-eigenvalue cycle (fission bank sort and resampling), the
+// single-event parity entry point and the TSL batch samplers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/minimc_amd.h"
+#define MMC_TRACKER_KERNELS
+#include "mmc_kernels.hpp"
+#include "mmc_mlp.hpp"
+
+using namespace mmc;
+
+#define HIPCHK(call)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) {                                            \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);    \
+      return MMC_ERR_HIP;                                              \
+    }                                                                  \
+  } while (0)
+
+static constexpr int kBlock = 256;
+
+struct mmc_ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mmc_options opt{};
+  DevWorld W{};
+  std::vector<void*> allocs;
+  bool has_tsl = false;
+  bool fixed_fission = false;  // fixed-source problem with fission children
+  int n_hash0 = 0;             // LDS entries staged by the CE kernels
+  // bank
+  uint32_t cap = 0;
+  RunState R{};
+  uint32_t* scan_tmp = nullptr;
+  uint32_t scan_cap = 0;
+  // tallies
+  int NB = 0;
+  double *S = nullptr, *S2 = nullptr;
+  std::vector<int> est_off, est_size;
+  std::vector<int> cell_material;  // host copy for argument checks
+  uint32_t *h_cnt = nullptr, *h_off = nullptr, *h_fill = nullptr;
+  uint32_t* rs_bin = nullptr;
+  double* rs_val = nullptr;
+  // k-eigenvalue
+  mmc_site* src = nullptr;
+  mmc_site* sites_sorted = nullptr;
+  uint32_t* fis_off = nullptr;
+  uint64_t N = 0;
+  int mesh_n[3] = {1, 1, 1};
+  double mesh_lo[3] = {-1e300, -1e300, -1e300};
+  double mesh_hi[3] = {1e300, 1e300, 1e300};
+  uint32_t* mesh_cnt = nullptr;
+  double* kpart = nullptr;
+  uint32_t kpart_cap = 0;
+  mmc_counters cnt{};
+  unsigned long long ev_host[kEvN] = {};
+  // surrogate
+  MlpState mlp{};
+};
+
+template <class T>
+static T* DevAlloc(mmc_ctx* ctx, size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+  ctx->allocs.push_back(p);
+  return static_cast<T*>(p);
+}
+template <class T>
+static T* DevUpload(mmc_ctx* ctx, const std::vector<T>& v) {
+  T* p = DevAlloc<T>(ctx, v.size());
+  if (p && !v.empty()) hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return p;
+}
+static inline uint32_t Blocks(uint64_t n, uint32_t cap = 4096) {
+  uint64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return uint32_t(std::min<uint64_t>(b, cap));
+}
+
+// ---------------------------------------------------------------------------
+// World flattening
+namespace {
+
+struct Flat {
+  std::vector<double> pool;
+  std::vector<int> ipool;
+  std::vector<DGrid> grids;
+  std::vector<std::vector<double>> grid_keys;
+  std::vector<DTable> tables;
+
+  int AddDoubles(const double* p, size_t n) {
+    const int off = int(pool.size());
+    pool.insert(pool.end(), p, p + n);
+    return off;
+  }
+  int AddGrid(const double* x, int n) {
+    for (size_t g = 0; g < grids.size(); g++)
+      if (grid_keys[g].size() == size_t(n) &&
+          std::memcmp(grid_keys[g].data(), x, sizeof(double) * n) == 0)
+        return int(g);
+    DGrid g{};
+    g.x_off = AddDoubles(x, n);
+    g.n = n;
+    g.n_hash = 0;
+    g.hash_off = 0;
+    if (n >= 64 && x[0] > 0) {
+      // log-spaced buckets; H[b] = upper_bound(x, edge_b) (host, exact)
+      int nh = 1;
+      while (nh < n / 4 && nh < 8192) nh <<= 1;
+      const double l0 = std::log(x[0]);
+      const double dl = (std::log(x[n - 1]) - l0) / nh;
+      g.log_x0 = l0;
+      g.inv_dlog = 1.0 / dl;
+      g.n_hash = nh;
+      g.hash_off = int(ipool.size());
+      for (int b = 0; b <= nh; b++) {
+        const double edge = b == 0 ? x[0] : (b == nh ? x[n - 1] : std::exp(l0 + b * dl));
+        ipool.push_back(int(std::upper_bound(x, x + n, edge) - x));
+      }
+      // ensure the extreme buckets cover everything strictly inside
+      ipool[g.hash_off] = 0;
+      ipool[g.hash_off + nh] = n;
+    }
+    grids.push_back(g);
+    grid_keys.emplace_back(x, x + n);
+    return int(grids.size()) - 1;
+  }
+  int AddTable(const mmc_table& t) {
+    if (t.n <= 0 || !t.x || !t.y) return -1;
+    DTable d{};
+    d.grid = AddGrid(t.x, t.n);
+    d.y_off = AddDoubles(t.y, t.n);
+    tables.push_back(d);
+    return int(tables.size()) - 1;
+  }
+};
+
+DBins MakeBins(Flat& F, const mmc_bins& b, bool present) {
+  DBins d{};
+  d.kind = present ? b.kind : MMC_BINS_NONE;
+  switch (d.kind) {
+    case MMC_BINS_LINSPACE:
+    case MMC_BINS_LOGSPACE:
+      d.n_bins = b.n + 2;
+      d.lo = b.lo;
+      d.hi = b.hi;
+      d.width = (b.hi - b.lo) / double(d.n_bins - 2);
+      d.base_log = std::log(d.kind == MMC_BINS_LOGSPACE && b.base > 0 ? b.base : 10.0);
+      break;
+    case MMC_BINS_BOUNDARIES:
+      d.n_bins = b.n + 1;
+      d.bounds_off = F.AddDoubles(b.bounds, b.n);
+      d.n_bounds = b.n;
+      break;
+    default:
+      d.n_bins = 1;
+  }
+  return d;
+}
+
+}  // namespace
+
+static int Fail(mmc_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+extern "C" int mmc_abi_version(void) { return MMC_ABI_VERSION; }
+
+extern "C" int mmc_device_count(int* n) {
+  if (!n) return MMC_ERR_ARG;
+  if (hipGetDeviceCount(n) != hipSuccess) {
+    *n = 0;
+    return MMC_ERR_NODEV;
+  }
+  return MMC_OK;
+}
+
+static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_desc* td) {
+  Flat F;
+  DevWorld& W = ctx->W;
+  W.energy_mode = wd->energy_mode;
+  W.G = wd->groups;
+  W.tracking = ctx->opt.tracking;
+  W.n_cells = wd->n_cells;
+  if (wd->n_cells <= 0 || wd->n_materials <= 0 || wd->n_nuclides <= 0)
+    return Fail(ctx, MMC_ERR_ARG, "world needs cells, materials and nuclides");
+  std::vector<DSurface> surf(wd->n_surfaces);
+  for (int i = 0; i < wd->n_surfaces; i++) {
+    const auto& s = wd->surfaces[i];
+    if (s.kind < 0 || s.kind > 4) return Fail(ctx, MMC_ERR_ARG, "bad surface kind");
+    surf[i] = DSurface{s.a, s.b, s.c, s.r, s.kind, 0};
+  }
+  std::vector<DCell> cells(wd->n_cells);
+  for (int i = 0; i < wd->n_cells; i++) {
+    const auto& c = wd->cells[i];
+    if (c.material >= wd->n_materials || c.field < 0 || c.field >= wd->n_fields ||
+        c.face_begin < 0 || c.face_begin + c.face_count > wd->n_faces)
+      return Fail(ctx, MMC_ERR_ARG, "bad cell " + std::to_string(i));
+    cells[i] = DCell{c.material, c.face_begin, c.face_count, c.field};
+    ctx->cell_material.push_back(c.material);
+  }
+  std::vector<DFace> faces(wd->n_faces);
+  for (int i = 0; i < wd->n_faces; i++) {
+    if (wd->faces[i].surface < 0 || wd->faces[i].surface >= wd->n_surfaces)
+      return Fail(ctx, MMC_ERR_ARG, "bad face surface");
+    faces[i] = DFace{wd->faces[i].surface, wd->faces[i].inside ? 1 : 0};
+  }
+  std::vector<DField> fields(wd->n_fields);
+  for (int i = 0; i < wd->n_fields; i++) {
+    const auto& f = wd->fields[i];
+    fields[i] = DField{f.kind, 0, f.c, f.gx, f.gy, f.gz, f.b,
+                       f.kind == MMC_FIELD_CONSTANT ? f.c : f.upper};
+  }
+  // Material::AssignNuclides normalises atom fractions (Material.cpp:82-92)
+  std::vector<DMaterial> mats(wd->n_materials);
+  std::vector<DComp> comps;
+  for (int i = 0; i < wd->n_materials; i++) {
+    const auto& m = wd->materials[i];
+    double sum = 0;
+    for (int k = 0; k < m.comp_count; k++) sum += wd->components[m.comp_begin + k].afrac;
+    mats[i] = DMaterial{m.aden, int(comps.size()), m.comp_count};
+    for (int k = 0; k < m.comp_count; k++) {
+      const auto& c = wd->components[m.comp_begin + k];
+      if (c.nuclide < 0 || c.nuclide >= wd->n_nuclides)
+        return Fail(ctx, MMC_ERR_ARG, "bad component nuclide");
+      comps.push_back(DComp{c.afrac / sum, c.nuclide, 0});
+    }
+  }
+  const int G = W.G;
+  const int NN = wd->n_nuclides;
+  std::vector<double> cap, sxs, fis, nub, tot, sp, cp;
+  std::vector<DCeNuclide> ce;
+  std::vector<DCeReaction> rx;
+  std::vector<DTsl> tsl;
+  std::vector<DTslPart> parts;
+  if (W.energy_mode == MMC_MULTIGROUP) {
+    if (G <= 0) return Fail(ctx, MMC_ERR_ARG, "multigroup needs groups > 0");
+    cap.assign(size_t(NN) * G, 0);
+    sxs = fis = nub = tot = cap;
+    sp.assign(size_t(NN) * G * G, 0);
+    cp = sp;
+    for (int n = 0; n < NN; n++) {
+      const auto& d = wd->mg[n];
+      for (int g = 0; g < G; g++) {
+        if (d.capture) cap[n * G + g] = d.capture[g];
+        if (d.fission_xs) fis[n * G + g] = d.fission_xs[g];
+        if (d.nubar && d.fission_xs) nub[n * G + g] = d.nubar[g];
+      }
+      // columns of the XML matrices: incoming group gp, outgoing g
+      // (Multigroup.cpp:124-147); normalised per column (:168-186)
+      auto column_probs = [&](const double* m, std::vector<double>& out, double* sums) {
+        for (int gp = 0; gp < G; gp++) {
+          double s = 0;
+          for (int g = 0; g < G; g++) s += m ? m[G * g + gp] : 0.0;
+          if (sums) sums[gp] = s;
+          for (int g = 0; g < G; g++) {
+            const double v = m ? m[G * g + gp] : 0.0;
+            out[(size_t(n) * G + gp) * G + g] = s == 0. ? v : v / s;
+          }
+        }
+      };
+      column_probs(d.scatter, sp, &sxs[n * G]);
+      column_probs(d.chi, cp, nullptr);
+      for (int g = 0; g < G; g++) {
+        double t = 0;  // CreateTotalXS (Multigroup.cpp:231-243), enum order
+        if (d.capture) t = t + cap[n * G + g];
+        if (d.scatter) t = t + sxs[n * G + g];
+        if (d.fission_xs) t = t + fis[n * G + g];
+        tot[n * G + g] = t;
+      }
+    }
+  } else {
+    for (int n = 0; n < NN; n++) {
+      const auto& d = wd->ce[n];
+      DCeNuclide dn{};
+      dn.awr = d.awr;
+      dn.total_temperature = d.total_temperature;
+      dn.has_total = d.has_total && d.total.n > 0;
+      dn.total = dn.has_total ? F.AddTable(d.total) : -1;
+      dn.rx_begin = int(rx.size());
+      dn.rx_count = d.n_reactions;
+      dn.fissile = 0;
+      int grid = dn.has_total ? F.tables[dn.total].grid : -2;
+      for (int k = 0; k < d.n_reactions; k++) {
+        const auto& r = d.reactions[k];
+        DCeReaction dr{};
+        dr.kind = r.kind;
+        dr.tsl = r.kind == MMC_RX_SCATTER ? r.tsl : -1;
+        if (dr.tsl >= wd->n_tsl) return Fail(ctx, MMC_ERR_ARG, "bad tsl index");
+        dr.xs = F.AddTable(r.xs);
+        if (dr.xs < 0) return Fail(ctx, MMC_ERR_ARG, "reaction without xs table");
+        dr.nubar = r.kind == MMC_RX_FISSION ? F.AddTable(r.nubar) : -1;
+        if (r.kind == MMC_RX_FISSION) {
+          if (dr.nubar < 0) return Fail(ctx, MMC_ERR_ARG, "fission without nubar");
+          dn.fissile = 1;
+        }
+        dr.chi_kind = r.chi_kind;
+        dr.temperature = r.temperature;
+        dr.watt_a = r.watt_a;
+        dr.watt_b = r.watt_b;
+        const int rg = F.tables[dr.xs].grid;
+        grid = grid == -2 ? rg : (grid == rg ? grid : -1);
+        rx.push_back(dr);
+      }
+      dn.grid = grid >= 0 ? grid : -1;
+      ce.push_back(dn);
+    }
+    for (int i = 0; i < wd->n_tsl; i++) {
+      const auto& d = wd->tsl[i];
+      DTsl t{};
+      t.majorant = F.AddTable(d.majorant);
+      if (t.majorant < 0 || d.n_E < 1 || d.n_T < 1 || d.rank < 1)
+        return Fail(ctx, MMC_ERR_ARG, "bad tsl tables");
+      t.E_axis_off = F.AddDoubles(d.total_E_axis, d.n_E);
+      t.nE = d.n_E;
+      t.E_modes_off = F.AddDoubles(d.total_E, size_t(d.n_E) * d.rank);
+      t.T_axis_off = F.AddDoubles(d.total_T_axis, d.n_T);
+      t.nT = d.n_T;
+      t.T_modes_off = F.AddDoubles(d.total_T, size_t(d.n_T) * d.rank);
+      t.S_off = F.AddDoubles(d.total_S, d.rank);
+      t.rank = d.rank;
+      auto add_parts = [&](const mmc_tsl_partition* p, int np, int& begin, int& xs_off,
+                           int& nxs, int& ends_off) {
+        begin = int(parts.size());
+        std::vector<double> xs;
+        std::vector<int> ends;
+        for (int k = 0; k < np; k++) {
+          DTslPart q{};
+          q.F_off = F.AddDoubles(p[k].F, p[k].n_F);
+          q.nF = p[k].n_F;
+          q.cdf_off = F.AddDoubles(p[k].cdf_modes, size_t(p[k].n_F) * p[k].rank);
+          q.S_off = F.AddDoubles(p[k].S, p[k].rank);
+          q.x_off = F.AddDoubles(p[k].x, p[k].n_x);
+          q.nx = p[k].n_x;
+          q.T_off = F.AddDoubles(p[k].T, p[k].n_T);
+          q.nT = p[k].n_T;
+          q.xt_off = F.AddDoubles(p[k].x_T_modes, size_t(p[k].n_x) * p[k].n_T * p[k].rank);
+          q.rank = p[k].rank;
+          parts.push_back(q);
+          xs.insert(xs.end(), p[k].x, p[k].x + p[k].n_x);
+          ends.push_back(int(xs.size()));
+        }
+        xs_off = F.AddDoubles(xs.data(), xs.size());
+        nxs = int(xs.size());
+        ends_off = int(F.ipool.size());
+        F.ipool.insert(F.ipool.end(), ends.begin(), ends.end());
+      };
+      add_parts(d.beta_partitions, d.n_beta_partitions, t.bp_begin, t.Es_off, t.nEs,
+                t.E_ends_off);
+      t.n_bp = d.n_beta_partitions;
+      add_parts(d.alpha_partitions, d.n_alpha_partitions, t.ap_begin, t.betas_off,
+                t.nbetas, t.beta_ends_off);
+      t.n_ap = d.n_alpha_partitions;
+      t.beta_cutoff = d.beta_cutoff;
+      t.alpha_cutoff = d.alpha_cutoff;
+      t.cutoff_energy = d.total_E_axis[d.n_E - 1];  // ThermalScattering.hpp:125-126
+      t.awr = 0;
+      tsl.push_back(t);
+    }
+    for (int n = 0; n < NN; n++)
+      for (int k = 0; k < ce[n].rx_count; k++)
+        if (rx[ce[n].rx_begin + k].tsl >= 0) tsl[rx[ce[n].rx_begin + k].tsl].awr = ce[n].awr;
+    ctx->has_tsl = !tsl.empty();
+    if (!F.grids.empty()) ctx->n_hash0 = F.grids[0].n_hash;
+  }
+  // estimators
+  std::vector<DEst> ests;
+  int nb = 0;
+  if (td) {
+    for (int i = 0; i < td->n_estimators; i++) {
+      const auto& e = td->estimators[i];
+      DEst d{};
+      d.kind = e.kind;
+      d.surface = e.surface;
+      d.has_dir = e.has_direction ? 1 : 0;
+      const double nrm = std::sqrt(e.u * e.u + e.v * e.v + e.w * e.w);
+      d.dx = e.u / nrm;
+      d.dy = e.v / nrm;
+      d.dz = e.w / nrm;
+      d.cosine = MakeBins(F, e.cosine, d.has_dir);
+      d.energy = MakeBins(F, e.energy, true);
+      int size = 0;
+      if (e.kind == MMC_EST_FLUX_MESH) {
+        size = 1;
+        for (int a = 0; a < 3; a++) {
+          d.mesh_n[a] = e.mesh_n[a] > 0 ? e.mesh_n[a] : 1;
+          d.mesh_lo[a] = e.mesh_lo[a];
+          d.mesh_hi[a] = e.mesh_hi[a];
+          size *= d.mesh_n[a];
+        }
+        W.n_est_flux++;
+      } else if (e.kind == MMC_EST_TSL_ALPHA || e.kind == MMC_EST_TSL_BETA) {
+        size = d.energy.n_bins;
+        W.n_est_tsl++;
+      } else if (e.kind == MMC_EST_CURRENT) {
+        if (e.surface < 0 || e.surface >= wd->n_surfaces)
+          return Fail(ctx, MMC_ERR_ARG, "current estimator surface out of range");
+        size = d.cosine.n_bins * d.energy.n_bins;
+        W.n_est_current++;
+      } else {
+        return Fail(ctx, MMC_ERR_ARG, "unknown estimator kind");
+      }
+      d.bin_off = nb;
+      ctx->est_off.push_back(nb);
+      ctx->est_size.push_back(size);
+      nb += size;
+      ests.push_back(d);
+    }
+  }
+  W.n_est = int(ests.size());
+  ctx->NB = nb;
+  if (F.pool.empty()) F.pool.push_back(0);
+  if (F.ipool.empty()) F.ipool.push_back(0);
+  W.surfaces = DevUpload(ctx, surf);
+  W.cells = DevUpload(ctx, cells);
+  W.faces = DevUpload(ctx, faces);
+  W.fields = DevUpload(ctx, fields);
+  W.mats = DevUpload(ctx, mats);
+  W.comps = DevUpload(ctx, comps);
+  W.mg_capture = DevUpload(ctx, cap);
+  W.mg_scatter_xs = DevUpload(ctx, sxs);
+  W.mg_fission = DevUpload(ctx, fis);
+  W.mg_nubar = DevUpload(ctx, nub);
+  W.mg_total = DevUpload(ctx, tot);
+  W.mg_scatter_probs = DevUpload(ctx, sp);
+  W.mg_chi_probs = DevUpload(ctx, cp);
+  W.ce = DevUpload(ctx, ce);
+  W.rx = DevUpload(ctx, rx);
+  W.tables = DevUpload(ctx, F.tables);
+  W.grids = DevUpload(ctx, F.grids);
+  W.tsl = DevUpload(ctx, tsl);
+  W.parts = DevUpload(ctx, parts);
+  W.pool = DevUpload(ctx, F.pool);
+  W.ipool = DevUpload(ctx, F.ipool);
+  W.ests = DevUpload(ctx, ests);
+  ctx->S = DevAlloc<double>(ctx, nb);
+  ctx->S2 = DevAlloc<double>(ctx, nb);
+  ctx->R.cycS = DevAlloc<double>(ctx, nb);
+  HIPCHK(hipMemset(ctx->S, 0, sizeof(double) * std::max(nb, 1)));
+  HIPCHK(hipMemset(ctx->S2, 0, sizeof(double) * std::max(nb, 1)));
+  HIPCHK(hipMemset(ctx->R.cycS, 0, sizeof(double) * std::max(nb, 1)));
+  // fixed-source problems with fission keep half the slots for children
+  if (W.energy_mode == MMC_MULTIGROUP) {
+    for (double v : fis) ctx->fixed_fission |= v > 0;
+  } else {
+    for (const auto& n : ce) ctx->fixed_fission |= n.fissile != 0;
+  }
+  return MMC_OK;
+}
+
+static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
+  if (ctx->cap >= cap) return MMC_OK;
+  if (ctx->cap) return Fail(ctx, MMC_ERR_ARG, "bank already sized");
+  ctx->cap = cap;
+  RunState& R = ctx->R;
+  DevBank& B = R.B;
+  B.x = DevAlloc<double>(ctx, cap);
+  B.y = DevAlloc<double>(ctx, cap);
+  B.z = DevAlloc<double>(ctx, cap);
+  B.u = DevAlloc<double>(ctx, cap);
+  B.v = DevAlloc<double>(ctx, cap);
+  B.w = DevAlloc<double>(ctx, cap);
+  B.E = DevAlloc<double>(ctx, cap);
+  B.rng = DevAlloc<uint32_t>(ctx, cap);
+  B.cell = DevAlloc<int32_t>(ctx, cap);
+  B.surf = DevAlloc<int32_t>(ctx, cap);
+  B.hist = DevAlloc<uint32_t>(ctx, cap);
+  B.steps = DevAlloc<uint32_t>(ctx, cap);
+  B.k_col = DevAlloc<double>(ctx, cap);
+  B.k_trk = DevAlloc<double>(ctx, cap);
+  B.k_abs = DevAlloc<double>(ctx, cap);
+  B.tslT = DevAlloc<double>(ctx, cap);
+  B.tslI = DevAlloc<int32_t>(ctx, cap);
+  R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
+  R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
+  R.QC = DevAlloc<uint32_t>(ctx, cap);
+  R.QX = DevAlloc<uint32_t>(ctx, cap);
+  R.QT = DevAlloc<uint32_t>(ctx, cap);
+  R.ring = DevAlloc<uint32_t>(ctx, kRing * kRingStride);
+  R.misc = DevAlloc<uint32_t>(ctx, 16);
+  R.ev = DevAlloc<unsigned long long>(ctx, kEvN);
+  R.capacity = cap;
+  R.rec_cap = cap * 4 + 1024;
+  R.rec_hist = DevAlloc<uint32_t>(ctx, R.rec_cap);
+  R.rec_bin = DevAlloc<uint32_t>(ctx, R.rec_cap);
+  R.rec_val = DevAlloc<double>(ctx, R.rec_cap);
+  ctx->rs_bin = DevAlloc<uint32_t>(ctx, R.rec_cap);
+  ctx->rs_val = DevAlloc<double>(ctx, R.rec_cap);
+  ctx->h_cnt = DevAlloc<uint32_t>(ctx, cap);
+  ctx->h_off = DevAlloc<uint32_t>(ctx, cap);
+  ctx->h_fill = DevAlloc<uint32_t>(ctx, cap);
+  R.fis_cnt = DevAlloc<uint32_t>(ctx, cap);
+  ctx->fis_off = DevAlloc<uint32_t>(ctx, cap);
+  ctx->scan_cap = (cap + 1023) / 1024 + 1;
+  ctx->scan_tmp = DevAlloc<uint32_t>(ctx, ctx->scan_cap);
+  ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
+  ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
+  if (!ctx->kpart || !B.tslI) return Fail(ctx, MMC_ERR_HIP, "out of device memory");
+  HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
+  R.max_steps = uint32_t(ctx->opt.max_steps ? std::min<uint64_t>(ctx->opt.max_steps, 0xffffffffull)
+                                            : 1000000ull);
+  return MMC_OK;
+}
+
+extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tallies,
+                          const mmc_options* options, mmc_ctx** out) {
+  if (!world || !out) return MMC_ERR_ARG;
+  *out = nullptr;
+  auto* ctx = new mmc_ctx();
+  if (options) ctx->opt = *options;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= ctx->opt.device) {
+    delete ctx;
+    return MMC_ERR_NODEV;
+  }
+  ctx->device = ctx->opt.device;
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return MMC_ERR_HIP;
+  }
+  const int rc = BuildWorld(ctx, world, tallies);
+  if (rc) {
+    *out = ctx;  // keep the message available through mmc_last_error
+    return rc;
+  }
+  uint64_t cap = ctx->opt.max_in_flight ? ctx->opt.max_in_flight : (1ull << 20);
+  cap = std::min<uint64_t>(std::max<uint64_t>(cap, 1024), 1ull << 28);
+  const int rb = AllocBank(ctx, uint32_t(cap));
+  *out = ctx;
+  return rb;
+}
+
+extern "C" void mmc_destroy(mmc_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->allocs) hipFree(p);
+  MlpFree(ctx->mlp);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* mmc_last_error(const mmc_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+// ---------------------------------------------------------------------------
+// scan (exclusive, u32) and small utility kernels
+__global__ void KScanBlock(const uint32_t* in, uint32_t* out, uint32_t* sums, uint32_t n) {
+  __shared__ uint32_t s[1024];
+  const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  const uint32_t v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+}
+__global__ void KScanTop(uint32_t* sums, uint32_t nb) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? sums[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = s[threadIdx.x] - v + carry;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+}
+__global__ void KScanAdd(uint32_t* out, const uint32_t* sums, uint32_t n) {
+  const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += sums[blockIdx.x];
+}
+static int Scan(mmc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+  const uint32_t nb = (n + 1023) / 1024;
+  if (nb + 1 > ctx->scan_cap) return Fail(ctx, MMC_ERR_OVERFLOW, "scan too large");
+  hipLaunchKernelGGL(KScanBlock, dim3(nb), dim3(1024), 0, ctx->stream, in, out,
+                     ctx->scan_tmp, n);
+  hipLaunchKernelGGL(KScanTop, dim3(1), dim3(1024), 0, ctx->stream, ctx->scan_tmp, nb);
+  hipLaunchKernelGGL(KScanAdd, dim3(nb), dim3(1024), 0, ctx->stream, out, ctx->scan_tmp, n);
+  return MMC_OK;
+}
+__global__ void KFillU32(uint32_t* p, uint32_t v, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void KFillF64(double* p, double v, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = v;
+}
+static void FillU32(mmc_ctx* ctx, uint32_t* p, uint32_t v, uint32_t n) {
+  hipLaunchKernelGGL(KFillU32, dim3(Blocks(n, 2048)), dim3(kBlock), 0, ctx->stream, p, v, n);
+}
+static void FillF64(mmc_ctx* ctx, double* p, double v, uint32_t n) {
+  hipLaunchKernelGGL(KFillF64, dim3(Blocks(n, 2048)), dim3(kBlock), 0, ctx->stream, p, v, n);
+}
+
+// ---- per-history tally commit (ScorableProxy::CommitHistory, Scorable.cpp:101-106)
+__global__ void KRecCount(const uint32_t* rh, uint32_t nrec, uint32_t* cnt) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrec) atomicAdd(cnt + rh[r], 1u);
+}
+__global__ void KRecPlace(const uint32_t* rh, const uint32_t* rb, const double* rv,
+                          uint32_t nrec, const uint32_t* off, uint32_t* fill,
+                          uint32_t* sb, double* sv) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const uint32_t h = rh[r];
+  const uint32_t pos = off[h] + atomicAdd(fill + h, 1u);
+  sb[pos] = rb[r];
+  sv[pos] = rv[r];
+}
+__global__ void KRecCommit(const uint32_t* cnt, const uint32_t* off, const uint32_t* sb,
+                           const double* sv, uint32_t nh, double* S, double* S2) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const uint32_t c = cnt[h];
+  const uint32_t o = off[h];
+  for (uint32_t i = 0; i < c; i++) {
+    const uint32_t b = sb[o + i];
+    bool seen = false;
+    for (uint32_t j = 0; j < i && !seen; j++) seen = sb[o + j] == b;
+    if (seen) continue;
+    double s = 0;
+    for (uint32_t j = i; j < c; j++)
+      if (sb[o + j] == b) s += sv[o + j];
+    unsafeAtomicAdd(S + b, s);
+    unsafeAtomicAdd(S2 + b, s * s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Event loop
+static int CheckMisc(mmc_ctx* ctx, const uint32_t* misc) {
+  const uint32_t e = misc[0];
+  if (!e) return MMC_OK;
+  if (e & kErrLost)
+    return Fail(ctx, MMC_ERR_LOST,
+                "Point does not belong to any Cell. Please checkthat all space is "
+                "either assigned a material or void.");
+  if (e & kErrResample)
+    return Fail(ctx, MMC_ERR_RESAMPLE,
+                "Number of thermal scattering resamples exceeded limit: 10");
+  if (e & kErrOverflow) return Fail(ctx, MMC_ERR_OVERFLOW, "bank, queue or record overflow");
+  if (e & kErrSteps) return Fail(ctx, MMC_ERR_STEPS, "history step limit exceeded");
+  if (e & kErrReaction)
+    return Fail(ctx, MMC_ERR_ARG, "no reaction selected after 1000 resamples");
+  return Fail(ctx, MMC_ERR_ARG, "alpha sampling outside the beta grid");
+}
+
+static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
+  const size_t lds = ctx->W.energy_mode == 1 && ctx->n_hash0 > 0
+                         ? sizeof(int) * (ctx->n_hash0 + 1) : 0;
+  hipLaunchKernelGGL(KAdvance, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
+                     iter, qin);
+  hipLaunchKernelGGL(KCross, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R, iter,
+                     qin);
+  hipLaunchKernelGGL(KCollide, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
+                     iter, qin);
+  if (ctx->has_tsl) {
+    if (ctx->opt.tsl_mode == MMC_TSL_SURROGATE && ctx->mlp.ready) {
+      const int rc = MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, grid, ctx->stream);
+      if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
+    } else {
+      hipLaunchKernelGGL(KTslPod, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
+                         iter, qin);
+    }
+  }
+  return MMC_OK;
+}
+
+// Runs event iterations until no particle is alive.  The live-queue length
+// is read back every `K` iterations; in between, kernels read queue lengths
+// from device memory and grid-stride over them.
+static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
+  const int K = ctx->opt.graph_iters > 0 ? ctx->opt.graph_iters : 8;
+  uint32_t bound = n0;
+  int iter = 0, qin = 0;
+  std::vector<uint32_t> ring(kRing * kRingStride);
+  uint32_t misc[16];
+  for (;;) {
+    const uint32_t grid = Blocks(bound);
+    for (int k = 0; k < K; k++) {
+      const int rc = LaunchIteration(ctx, iter, qin, grid);
+      if (rc) return rc;
+      iter++;
+      qin ^= 1;
+    }
+    HIPCHK(hipMemcpyAsync(ring.data(), ctx->R.ring, sizeof(uint32_t) * ring.size(),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cnt.iterations += K;
+    const int rc = CheckMisc(ctx, misc);
+    if (rc) return rc;
+    const uint32_t live = ring[((iter - 1 + kRing) % kRing) * kRingStride + kQ];
+    if (live == 0) break;
+    bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
+  }
+  // the next run starts with ring slot -1 = kRing-1
+  if (iter % kRing != 0) {
+    // keep the ring consistent for the next call: it is fully reset there
+  }
+  return MMC_OK;
+}
+
+static int ResetRun(mmc_ctx* ctx) {
+  HIPCHK(hipMemsetAsync(ctx->R.ring, 0, sizeof(uint32_t) * kRing * kRingStride, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->R.misc, 0, sizeof(uint32_t) * 16, ctx->stream));
+  return MMC_OK;
+}
+
+static int ReadEvents(mmc_ctx* ctx) {
+  HIPCHK(hipMemcpyAsync(ctx->ev_host, ctx->R.ev, sizeof(ctx->ev_host), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->cnt.advances = ctx->ev_host[kEvAdvance];
+  ctx->cnt.xs_lookups = ctx->ev_host[kEvAdvance];
+  ctx->cnt.collisions = ctx->ev_host[kEvCollide];
+  ctx->cnt.crossings = ctx->ev_host[kEvCross];
+  ctx->cnt.virtual_collisions = ctx->ev_host[kEvVirtual];
+  ctx->cnt.tsl_collisions = ctx->ev_host[kEvTsl];
+  ctx->cnt.tsl_fallbacks = ctx->ev_host[kEvFallback];
+  ctx->cnt.tail_particles = ctx->ev_host[kEvTail];
+  return MMC_OK;
+}
+
+extern "C" int mmc_run_fixed_source(mmc_ctx* ctx, const mmc_source_desc* source,
+                                    uint64_t seed0, uint64_t first, uint64_t n) {
+  if (!ctx || !source) return MMC_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  ctx->R.keig = 0;
+  const uint32_t chunk_cap = ctx->fixed_fission ? ctx->cap / 2 : ctx->cap;
+  uint64_t done = 0;
+  while (done < n) {
+    const uint32_t m = uint32_t(std::min<uint64_t>(n - done, chunk_cap));
+    const uint64_t f = first + done;
+    if (f + m > 0xffffffffull) return Fail(ctx, MMC_ERR_ARG, "history index exceeds 2^32");
+    ctx->R.chunk_first = uint32_t(f);
+    int rc = ResetRun(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(KSourceFixed, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       ctx->stream, ctx->W, ctx->R, *source, seed0, uint32_t(f), m);
+    rc = RunEvents(ctx, m);
+    if (rc) return rc;
+    // commit this chunk's per-history scores
+    uint32_t misc[16];
+    HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const uint32_t nrec = misc[1];
+    if (nrec > 0) {
+      FillU32(ctx, ctx->h_cnt, 0, m);
+      FillU32(ctx, ctx->h_fill, 0, m);
+      hipLaunchKernelGGL(KRecCount, dim3(Blocks(nrec, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                         ctx->R.rec_hist, nrec, ctx->h_cnt);
+      rc = Scan(ctx, ctx->h_cnt, ctx->h_off, m);
+      if (rc) return rc;
+      hipLaunchKernelGGL(KRecPlace, dim3(Blocks(nrec, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                         ctx->R.rec_hist, ctx->R.rec_bin, ctx->R.rec_val, nrec, ctx->h_off,
+                         ctx->h_fill, ctx->rs_bin, ctx->rs_val);
+      hipLaunchKernelGGL(KRecCommit, dim3(Blocks(m, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                         ctx->h_cnt, ctx->h_off, ctx->rs_bin, ctx->rs_val, m, ctx->S, ctx->S2);
+    }
+    done += m;
+    ctx->cnt.histories += m;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int rc = ReadEvents(ctx);
+  ctx->cnt.transport_seconds +=
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// k-eigenvalue
+__global__ void KInitSites(const mmc_source_desc s, mmc_site* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t g = LcgSeed(uint64_t(i) + 1);  // seeds 1..N (KEigenvalue.cpp:31-33)
+  double q[3] = {s.x, s.y, s.z};
+  for (int a = 0; a < 3; a++)
+    if (s.box_hi[a] > s.box_lo[a]) q[a] = Uniform(g, s.box_lo[a], s.box_hi[a]);
+  V3 d;
+  if (s.dir_kind == 1) {
+    d = IsotropicDirection(g);
+  } else if (s.dir_kind == 2) {
+    const double mu = sqrt(Canonical(g));
+    const double phi = Uniform(g, 0., 2 * kPi);
+    d = Rotate(Normalized(V3{s.u, s.v, s.w}), mu, phi);
+  } else {
+    d = Normalized(V3{s.u, s.v, s.w});
+  }
+  mmc_site site;
+  site.x = q[0]; site.y = q[1]; site.z = q[2];
+  site.u = d.x; site.v = d.y; site.w = d.z;
+  site.E = s.energy_kind == 1 ? Uniform(g, s.E, s.E_hi) : s.E;
+  site.seed = LcgNext(g);
+  site.pad = 0;
+  site.key = i;
+  out[i] = site;
+}
+
+extern "C" int mmc_set_entropy_mesh(mmc_ctx* ctx, const int32_t n[3], const double lo[3],
+                                    const double hi[3]) {
+  if (!ctx || !n || !lo || !hi) return MMC_ERR_ARG;
+  for (int a = 0; a < 3; a++) {
+    if (n[a] < 1) return Fail(ctx, MMC_ERR_ARG, "entropy mesh needs n >= 1");
+    ctx->mesh_n[a] = n[a];
+    ctx->mesh_lo[a] = lo[a];
+    ctx->mesh_hi[a] = hi[a];
+  }
+  return MMC_OK;
+}
+
+static int EnsureKeig(mmc_ctx* ctx, uint64_t n) {
+  if (n > ctx->cap) return Fail(ctx, MMC_ERR_ARG, "k-eigenvalue source larger than max_in_flight");
+  if (!ctx->src) {
+    ctx->src = DevAlloc<mmc_site>(ctx, ctx->cap);
+    ctx->sites_sorted = DevAlloc<mmc_site>(ctx, size_t(ctx->cap) * 8);
+    ctx->R.sites = DevAlloc<mmc_site>(ctx, size_t(ctx->cap) * 8);
+    ctx->R.site_cap = ctx->cap * 8;
+    ctx->mesh_cnt = DevAlloc<uint32_t>(ctx, 1u << 20);
+    if (!ctx->mesh_cnt) return Fail(ctx, MMC_ERR_HIP, "out of device memory");
+  }
+  return MMC_OK;
+}
+
+extern "C" int mmc_keig_init(mmc_ctx* ctx, const mmc_source_desc* source, uint64_t n) {
+  if (!ctx || !source || n == 0) return MMC_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = EnsureKeig(ctx, n);
+  if (rc) return rc;
+  ctx->N = n;
+  hipLaunchKernelGGL(KInitSites, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                     *source, ctx->src, uint32_t(n));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MMC_OK;
+}
+
+__global__ void KSitePlace(const mmc_site* in, uint32_t m, const uint32_t* off, mmc_site* out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const mmc_site s = in[k];
+  out[off[s.key / 64] + uint32_t(s.key % 64)] = s;
+}
+__device__ __forceinline__ uint64_t Remix(uint64_t s, uint64_t c) {
+  uint64_t z = s + c * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return z % 2147483647ULL;
+}
+// source j <- bank[(j M) / N]; extra copies of a site get Remix(seed, copy).
+__global__ void KResample(const mmc_site* bank, uint64_t M, uint64_t N, mmc_site* src) {
+  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint64_t idx = (j * M) / N;
+  const uint64_t j0 = (idx * N + M - 1) / M;
+  const uint64_t copy = j - j0;
+  mmc_site s = bank[idx];
+  if (copy) s.seed = uint32_t(Remix(s.seed, copy));
+  s.key = j;
+  src[j] = s;
+}
+__global__ void KMeshCount(const mmc_site* s, uint32_t m, int nx, int ny, int nz, double lx,
+                           double ly, double lz, double hx, double hy, double hz,
+                           uint32_t* cnt) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const double q[3] = {s[k].x, s[k].y, s[k].z};
+  const double lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+  const int nn[3] = {nx, ny, nz};
+  int idx = 0;
+  for (int a = 0; a < 3; a++) {
+    const double f = (q[a] - lo[a]) / (hi[a] - lo[a]);
+    if (!(f >= 0 && f < 1)) return;
+    int i = int(f * nn[a]);
+    if (i >= nn[a]) i = nn[a] - 1;
+    idx = idx * nn[a] + i;
+  }
+  atomicAdd(cnt + idx, 1u);
+}
+// deterministic per-block sums of the per-history k accumulators
+__global__ void KKSum(const double* a, const double* b, const double* c, uint32_t n,
+                      double* part) {
+  __shared__ double s[3][kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  s[0][threadIdx.x] = i < n ? a[i] : 0;
+  s[1][threadIdx.x] = i < n ? b[i] : 0;
+  s[2][threadIdx.x] = i < n ? c[i] : 0;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int q = 0; q < 3; q++) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = s[threadIdx.x][0];
+}
+__global__ void KFoldCycle(double* S, double* S2, const double* cyc, uint32_t nb, double inv_n) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double v = cyc[b] * inv_n;
+  S[b] += v;
+  S2[b] += v * v;
+}
+
+extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
+  if (!ctx || ctx->N == 0) return MMC_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t N = uint32_t(ctx->N);
+  RunState& R = ctx->R;
+  R.keig = 1;
+  int rc = ResetRun(ctx);
+  if (rc) return rc;
+  if (ctx->NB) FillF64(ctx, R.cycS, 0.0, ctx->NB);
+  hipLaunchKernelGGL(KSourceSites, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                     ctx->W, R, ctx->src, N);
+  rc = RunEvents(ctx, N);
+  if (rc) return rc;
+  uint32_t misc[16];
+  HIPCHK(hipMemcpyAsync(misc, R.misc, sizeof(misc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const uint32_t M = misc[2];
+  // sort the bank by key = history * 64 + child
+  rc = Scan(ctx, R.fis_cnt, ctx->fis_off, N);
+  if (rc) return rc;
+  if (M) hipLaunchKernelGGL(KSitePlace, dim3(Blocks(M, 1u << 30)), dim3(kBlock), 0,
+                            ctx->stream, R.sites, M, ctx->fis_off, ctx->sites_sorted);
+  // entropy
+  const uint32_t nmesh = uint32_t(ctx->mesh_n[0]) * ctx->mesh_n[1] * ctx->mesh_n[2];
+  if (nmesh > (1u << 20)) return Fail(ctx, MMC_ERR_ARG, "entropy mesh too large");
+  FillU32(ctx, ctx->mesh_cnt, 0, nmesh);
+  if (M) hipLaunchKernelGGL(KMeshCount, dim3(Blocks(M, 1u << 30)), dim3(kBlock), 0,
+                            ctx->stream, ctx->sites_sorted, M, ctx->mesh_n[0], ctx->mesh_n[1],
+                            ctx->mesh_n[2], ctx->mesh_lo[0], ctx->mesh_lo[1], ctx->mesh_lo[2],
+                            ctx->mesh_hi[0], ctx->mesh_hi[1], ctx->mesh_hi[2], ctx->mesh_cnt);
+  const uint32_t nbk = Blocks(N, 1u << 30);
+  hipLaunchKernelGGL(KKSum, dim3(nbk), dim3(kBlock), 0, ctx->stream, R.B.k_col, R.B.k_trk,
+                     R.B.k_abs, N, ctx->kpart);
+  if (active && ctx->NB)
+    hipLaunchKernelGGL(KFoldCycle, dim3(Blocks(ctx->NB, 1u << 30)), dim3(kBlock), 0,
+                       ctx->stream, ctx->S, ctx->S2, R.cycS, uint32_t(ctx->NB), 1.0 / N);
+  std::vector<uint32_t> mc(nmesh);
+  std::vector<double> kp(size_t(nbk) * 3);
+  HIPCHK(hipMemcpyAsync(mc.data(), ctx->mesh_cnt, sizeof(uint32_t) * nmesh,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(kp.data(), ctx->kpart, sizeof(double) * kp.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double kc = 0, kt = 0, ka = 0;
+  for (uint32_t b = 0; b < nbk; b++) {
+    kc += kp[b * 3];
+    kt += kp[b * 3 + 1];
+    ka += kp[b * 3 + 2];
+  }
+  uint64_t tot = 0;
+  for (uint32_t c : mc) tot += c;
+  double H = 0;
+  for (uint32_t c : mc)
+    if (c) {
+      const double p = double(c) / double(tot);
+      H -= p * std::log2(p);
+    }
+  ctx->cnt.histories += N;
+  ctx->cnt.fission_sites += M;
+  if (out) {
+    out->k_analog = double(M) / double(N);
+    out->k_collision = kc / N;
+    out->k_track = kt / N;
+    out->k_absorption = ka / N;
+    out->entropy = H;
+    out->n_sites = M;
+  }
+  if (M == 0) return Fail(ctx, MMC_ERR_ARG, "no fission sites banked");
+  hipLaunchKernelGGL(KResample, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                     ctx->sites_sorted, uint64_t(M), uint64_t(N), ctx->src);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  rc = ReadEvents(ctx);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  ctx->cnt.transport_seconds += secs;
+  if (out) out->seconds = secs;
+  return rc;
+}
+
+extern "C" int mmc_get_source(mmc_ctx* ctx, mmc_site* out, uint64_t cap, uint64_t* n) {
+  if (!ctx || !n) return MMC_ERR_ARG;
+  *n = ctx->N;
+  if (out && ctx->N) {
+    HIPCHK(hipMemcpy(out, ctx->src, sizeof(mmc_site) * std::min<uint64_t>(cap, ctx->N),
+                     hipMemcpyDeviceToHost));
+  }
+  return MMC_OK;
+}
+extern "C" int mmc_set_source(mmc_ctx* ctx, const mmc_site* sites, uint64_t n) {
+  if (!ctx || !sites || n == 0) return MMC_ERR_ARG;
+  int rc = EnsureKeig(ctx, n);
+  if (rc) return rc;
+  ctx->N = n;
+  HIPCHK(hipMemcpy(ctx->src, sites, sizeof(mmc_site) * n, hipMemcpyHostToDevice));
+  return MMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tallies and counters
+extern "C" int mmc_tally_size(mmc_ctx* ctx, int e, uint64_t* n) {
+  if (!ctx || !n || e < 0 || e >= int(ctx->est_off.size())) return MMC_ERR_ARG;
+  *n = uint64_t(ctx->est_size[e]);
+  return MMC_OK;
+}
+extern "C" int mmc_get_tallies(mmc_ctx* ctx, int e, double* S, double* S2, uint64_t n) {
+  if (!ctx || e < 0 || e >= int(ctx->est_off.size()) || n != uint64_t(ctx->est_size[e]))
+    return MMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (S)
+    HIPCHK(hipMemcpy(S, ctx->S + ctx->est_off[e], sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (S2)
+    HIPCHK(hipMemcpy(S2, ctx->S2 + ctx->est_off[e], sizeof(double) * n, hipMemcpyDeviceToHost));
+  return MMC_OK;
+}
+extern "C" int mmc_reset_tallies(mmc_ctx* ctx) {
+  if (!ctx) return MMC_ERR_ARG;
+  if (ctx->NB) {
+    FillF64(ctx, ctx->S, 0.0, ctx->NB);
+    FillF64(ctx, ctx->S2, 0.0, ctx->NB);
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MMC_OK;
+}
+extern "C" int mmc_get_counters(mmc_ctx* ctx, mmc_counters* out) {
+  if (!ctx || !out) return MMC_ERR_ARG;
+  *out = ctx->cnt;
+  return MMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single-event parity entry point
+struct EvalSoA {
+  double *x, *y, *z, *u, *v, *w, *E, *out0, *out1;
+  uint32_t* rng;
+  int32_t *cell, *surface, *event, *iout;
+};
+
+__global__ void KEval(DevWorld W, int kind, EvalSoA s, uint32_t n, uint32_t* err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V3 pos{s.x[i], s.y[i], s.z[i]};
+  V3 dir{s.u[i], s.v[i], s.w[i]};
+  double E = s.E[i];
+  uint32_t g = s.rng[i];
+  const int cell = s.cell[i];
+  switch (kind) {
+    case MMC_EVAL_FIND_CELL:
+      s.cell[i] = FindCell(W, pos);
+      break;
+    case MMC_EVAL_NEAREST_SURFACE: {
+      int sf;
+      s.out0[i] = NearestSurface(W, cell, pos, dir, sf);
+      s.iout[i] = sf;
+      break;
+    }
+    case MMC_EVAL_XS_TOTAL: {
+      const DCell c = W.cells[cell];
+      const DField f = W.fields[c.field];
+      s.out0[i] = MaterialTotal(W, c.material, E, FieldAt(f, pos), false, nullptr);
+      s.out1[i] = MaterialTotal(W, c.material, E, f.kind == 0 ? f.c : f.upper, true, nullptr);
+      break;
+    }
+    case MMC_EVAL_COLLISION_DISTANCE: {
+      const DCell c = W.cells[cell];
+      const DField f = W.fields[c.field];
+      const bool delta = W.tracking == 1;
+      const double sig = MaterialTotal(W, c.material, E,
+                                       delta ? (f.kind == 0 ? f.c : f.upper) : FieldAt(f, pos),
+                                       delta, nullptr);
+      s.out0[i] = -log(1.0 - Canonical(g)) / (W.mats[c.material].aden * sig);
+      break;
+    }
+    case MMC_EVAL_COLLIDE: {
+      int tsl = -1;
+      CollideOut o = Collide(W, nullptr, 0, pos, dir, E, g, cell, nullptr, nullptr, nullptr,
+                             nullptr, &tsl);
+      if (o.tsl) {
+        double a, b;
+        const double T = FieldAt(W.fields[W.cells[cell].field], pos);
+        const uint32_t e = TslScatterPod(W, W.tsl[tsl], g, T, dir, E, a, b);
+        if (e) atomicOr(err, e);
+      }
+      if (o.event >= 0) s.event[i] = o.event;
+      if (o.children < 0) atomicOr(err, kErrReaction);
+      s.iout[i] = o.children < 0 ? 0 : o.children;
+      break;
+    }
+    case MMC_EVAL_SCATTER_ROTATE: {
+      const double phi = 2 * kPi * Canonical(g);
+      dir = Rotate(dir, s.out0[i], phi);
+      E = s.out1[i];
+      break;
+    }
+    case MMC_EVAL_TSL_SAMPLE: {
+      double a = 0, b = 0;
+      const DTsl t = W.tsl[0];
+      uint32_t e = SampleBeta(W, t, g, E, s.out0[i], b) ? 0u : uint32_t(kErrResample);
+      if (!e) e = SampleAlpha(W, t, g, b, E, s.out0[i], a);
+      if (e) {
+        atomicOr(err, e);
+        a = b = __builtin_nan("");
+      }
+      s.out0[i] = a;
+      s.out1[i] = b;
+      break;
+    }
+    default:
+      break;
+  }
+  s.u[i] = dir.x;
+  s.v[i] = dir.y;
+  s.w[i] = dir.z;
+  s.E[i] = E;
+  s.rng[i] = g;
+}
+
+extern "C" int mmc_eval_event(mmc_ctx* ctx, int kind, mmc_particle_soa* p, uint64_t n) {
+  if (!ctx || !p || kind < 0 || kind > MMC_EVAL_TSL_SAMPLE) return MMC_ERR_ARG;
+  if (n == 0) return MMC_OK;
+  if (n > (1u << 26)) return Fail(ctx, MMC_ERR_ARG, "batch too large");
+  if (kind == MMC_EVAL_TSL_SAMPLE && !ctx->has_tsl) return Fail(ctx, MMC_ERR_ARG, "no tsl");
+  HIPCHK(hipSetDevice(ctx->device));
+  // host-side range checks of the cell indices the kernels will use
+  if (kind != MMC_EVAL_FIND_CELL && kind != MMC_EVAL_SCATTER_ROTATE &&
+      kind != MMC_EVAL_TSL_SAMPLE) {
+    const bool needs_material = kind != MMC_EVAL_NEAREST_SURFACE;
+    for (uint64_t i = 0; i < n; i++) {
+      const int c = p->cell[i];
+      if (c < 0 || c >= ctx->W.n_cells) return Fail(ctx, MMC_ERR_ARG, "cell out of range");
+      if (needs_material && ctx->cell_material[c] < 0)
+        return Fail(ctx, MMC_ERR_ARG, "event needs a material cell");
+    }
+  }
+  double* dbuf = nullptr;
+  uint32_t* ibuf = nullptr;
+  HIPCHK(hipMalloc(&dbuf, sizeof(double) * n * 9));
+  HIPCHK(hipMalloc(&ibuf, sizeof(uint32_t) * (n * 5 + 1)));
+  double* hd[9] = {p->x, p->y, p->z, p->u, p->v, p->w, p->E, p->out0, p->out1};
+  EvalSoA s;
+  double** sd[9] = {&s.x, &s.y, &s.z, &s.u, &s.v, &s.w, &s.E, &s.out0, &s.out1};
+  for (int k = 0; k < 9; k++) {
+    *sd[k] = dbuf + n * k;
+    if (hd[k]) hipMemcpy(*sd[k], hd[k], sizeof(double) * n, hipMemcpyHostToDevice);
+    else hipMemset(*sd[k], 0, sizeof(double) * n);
+  }
+  s.rng = ibuf;
+  s.cell = reinterpret_cast<int32_t*>(ibuf + n);
+  s.surface = reinterpret_cast<int32_t*>(ibuf + 2 * n);
+  s.event = reinterpret_cast<int32_t*>(ibuf + 3 * n);
+  s.iout = reinterpret_cast<int32_t*>(ibuf + 4 * n);
+  uint32_t* err = ibuf + 5 * n;
+  const void* hi[4] = {p->rng, p->cell, p->surface, p->event};
+  void* di[4] = {s.rng, s.cell, s.surface, s.event};
+  for (int k = 0; k < 4; k++) {
+    if (hi[k]) hipMemcpy(di[k], hi[k], sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    else hipMemset(di[k], 0, sizeof(uint32_t) * n);
+  }
+  hipMemset(s.iout, 0, sizeof(uint32_t) * n);
+  hipMemset(err, 0, sizeof(uint32_t));
+  hipLaunchKernelGGL(KEval, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream, ctx->W,
+                     kind, s, uint32_t(n), err);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < 9; k++)
+    if (hd[k]) hipMemcpy(hd[k], *sd[k], sizeof(double) * n, hipMemcpyDeviceToHost);
+  void* ho[5] = {p->rng, p->cell, p->surface, p->event, p->iout};
+  void* dd[5] = {s.rng, s.cell, s.surface, s.event, s.iout};
+  for (int k = 0; k < 5; k++)
+    if (ho[k]) hipMemcpy(ho[k], dd[k], sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+  uint32_t herr = 0;
+  hipMemcpy(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  hipFree(dbuf);
+  hipFree(ibuf);
+  if (e != hipSuccess) return Fail(ctx, MMC_ERR_HIP, hipGetErrorString(e));
+  uint32_t misc[16] = {herr};
+  return CheckMisc(ctx, misc);
+}
+
+// Source::Sample for histories seed0 + i, i < n, into host SoA.
+__global__ void KEvalSource(DevWorld W, mmc_source_desc s, uint64_t seed0, EvalSoA o,
+                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t g = LcgSeed(seed0 + i);
+  double q[3] = {s.x, s.y, s.z};
+  for (int a = 0; a < 3; a++)
+    if (s.box_hi[a] > s.box_lo[a]) q[a] = Uniform(g, s.box_lo[a], s.box_hi[a]);
+  V3 d;
+  if (s.dir_kind == 1) {
+    d = IsotropicDirection(g);
+  } else if (s.dir_kind == 2) {
+    const double mu = sqrt(Canonical(g));
+    const double phi = Uniform(g, 0., 2 * kPi);
+    d = Rotate(Normalized(V3{s.u, s.v, s.w}), mu, phi);
+  } else {
+    d = Normalized(V3{s.u, s.v, s.w});
+  }
+  o.x[i] = q[0]; o.y[i] = q[1]; o.z[i] = q[2];
+  o.u[i] = d.x; o.v[i] = d.y; o.w[i] = d.z;
+  o.E[i] = s.energy_kind == 1 ? Uniform(g, s.E, s.E_hi) : s.E;
+  o.rng[i] = LcgSeed(LcgNext(g));
+  o.cell[i] = FindCell(W, V3{q[0], q[1], q[2]});
+}
+
+extern "C" int mmc_sample_source(mmc_ctx* ctx, const mmc_source_desc* src, uint64_t seed0,
+                                 uint64_t n, mmc_particle_soa* p) {
+  if (!ctx || !src || !p || n == 0 || n > (1u << 26)) return MMC_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  double* dbuf = nullptr;
+  uint32_t* ibuf = nullptr;
+  HIPCHK(hipMalloc(&dbuf, sizeof(double) * n * 7));
+  HIPCHK(hipMalloc(&ibuf, sizeof(uint32_t) * n * 2));
+  EvalSoA s{};
+  double** sd[7] = {&s.x, &s.y, &s.z, &s.u, &s.v, &s.w, &s.E};
+  for (int k = 0; k < 7; k++) *sd[k] = dbuf + n * k;
+  s.rng = ibuf;
+  s.cell = reinterpret_cast<int32_t*>(ibuf + n);
+  hipLaunchKernelGGL(KEvalSource, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                     ctx->W, *src, seed0, s, uint32_t(n));
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  double* hd[7] = {p->x, p->y, p->z, p->u, p->v, p->w, p->E};
+  for (int k = 0; k < 7; k++)
+    if (hd[k]) hipMemcpy(hd[k], *sd[k], sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (p->rng) hipMemcpy(p->rng, s.rng, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+  if (p->cell) hipMemcpy(p->cell, s.cell, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+  hipFree(dbuf);
+  hipFree(ibuf);
+  if (e != hipSuccess) return Fail(ctx, MMC_ERR_HIP, hipGetErrorString(e));
+  return MMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch POD sampling of (alpha, beta) on device-resident inputs.
+__global__ void KTslBatch(DevWorld W, int tsl, uint32_t n, const double* E, const double* T,
+                          uint32_t* rng, double* alpha, double* beta, uint32_t* err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t g = rng[i];
+  double a = 0, b = 0;
+  const DTsl t = W.tsl[tsl];
+  uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
+  if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);
+  if (e) {
+    atomicOr(err, e);
+    a = b = __builtin_nan("");
+  }
+  alpha[i] = a;
+  beta[i] = b;
+  rng[i] = g;
+}
+
+extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
+                                  const double* T, uint32_t* rng, double* alpha,
+                                  double* beta) {
+  if (!ctx || !ctx->has_tsl || tsl < 0 || !E || !T || !rng || !alpha || !beta)
+    return MMC_ERR_ARG;
+  if (n == 0) return MMC_OK;
+  if (n > 0xffffffffull) return Fail(ctx, MMC_ERR_ARG, "batch too large");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemsetAsync(ctx->R.misc, 0, sizeof(uint32_t), ctx->stream));
+  hipLaunchKernelGGL(KTslBatch, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
+                     ctx->W, tsl, uint32_t(n), E, T, rng, alpha, beta, ctx->R.misc);
+  uint32_t misc[16] = {0};
+  HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CheckMisc(ctx, misc);
+}
+
+hipStream_t MmcStream(mmc_ctx* ctx) { return ctx->stream; }
+MlpState& MmcMlp(mmc_ctx* ctx) { return ctx->mlp; }
+const DevWorld& MmcWorld(mmc_ctx* ctx) { return ctx->W; }
+int MmcFail(mmc_ctx* ctx, int code, const char* msg) { return Fail(ctx, code, msg); }
+int MmcDevice(mmc_ctx* ctx) { return ctx->device; }

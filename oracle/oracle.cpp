@@ -61,8 +61,10 @@ enum ErrBits : int {
   kErrLost = 1,
   kErrResample = 2,
   kErrSteps = 4,
-  kErrAlphaRange = 8
+  kErrAlphaRange = 8,
+  kErrReaction = 32
 };
+constexpr int kMaxReactionResamples = 1000;
 
 // ---- Point / Direction (src/Point.cpp) -----------------------------------
 struct Pt {
@@ -688,6 +690,7 @@ void MgInteract(Ctx& c, const MgNuclide& n, Particle& p) {
 // ThermalScattering::SampleBeta (ThermalScattering.cpp:271-338)
 bool SampleBeta(const Tsl& t, Lcg& g, double E, double T, double& beta) {
   const size_t e_hi = UpperBound(t.Es, E);
+  if (e_hi >= t.Es.size()) return false;  // E above the beta grid (:278 asserts)
   const double r =
       e_hi != 0 ? (E - t.Es[e_hi - 1]) / (t.Es[e_hi] - t.Es[e_hi - 1]) : 1;
   const size_t e_s = r <= Canonical(g) ? e_hi - 1 : e_hi;
@@ -851,7 +854,8 @@ int CeScatter(Ctx& c, const CeNuclide& n, const CeReaction& r, Particle& p) {
 int CeInteract(Ctx& c, const CeNuclide& n, Particle& p) {
   World& w = *c.w;
   const double T = CellT(w, p);
-  for (;;) {
+  for (int attempt = 0;; attempt++) {
+    if (attempt >= kMaxReactionResamples) return kErrReaction;
     const double threshold = Canonical(p.rng) * CeTotal(w, n, p.E, T, false);
     double acc = 0;
     for (const auto& r : n.rx) {
@@ -1422,6 +1426,7 @@ static int ErrToStatus(orc_world* ow, int err) {
     return Fail(ow, MMC_ERR_RESAMPLE,
                 "Number of thermal scattering resamples exceeded limit: 10");
   if (err & kErrSteps) return Fail(ow, MMC_ERR_STEPS, "history step limit exceeded");
+  if (err & kErrReaction) return Fail(ow, MMC_ERR_ARG, "no reaction selected after 1000 resamples");
   return Fail(ow, MMC_ERR_ARG, "alpha sampling outside the beta grid");
 }
 
