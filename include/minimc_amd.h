@@ -272,7 +272,17 @@ typedef struct mmc_options {
                              kernel finishes a batch (0 = default,
                              -1 = never)                                */
   uint64_t max_steps;   /* per-history event limit (0 = 1e6)            */
+  int32_t quirks;       /* mmc_quirk bits (0 = corrected behaviour)      */
+  int32_t pad;
 } mmc_options;
+/* Reference behaviours that are reproduced only on request.
+ * MMC_QUIRK_NUDGE_CROSSING: after a surface crossing the new cell is the
+ *   first cell containing the point nudged by 10 epsilon (absolute;
+ *   TransportMethod.cpp:68, Constants.hpp:15).  At |x| >~ 1 cm the nudge can
+ *   vanish in rounding and leave the particle in the old cell, beyond its
+ *   surface.  Without the bit the crossed surface's sense is flipped
+ *   explicitly when the new cell is searched. */
+enum mmc_quirk { MMC_QUIRK_NUDGE_CROSSING = 1 };
 
 /* ---- particle state for single-event parity checks ----------------------
  * Structure of arrays, n entries each.  For multigroup, E holds the group

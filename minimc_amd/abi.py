@@ -25,6 +25,7 @@ MMC_DIR_CONSTANT, MMC_DIR_ISOTROPIC, MMC_DIR_ISOTROPIC_FLUX = 0, 1, 2
 MMC_E_CONSTANT, MMC_E_UNIFORM = 0, 1
 MMC_SURFACE_TRACKING, MMC_CELL_DELTA_TRACKING = 0, 1
 MMC_TSL_POD, MMC_TSL_SURROGATE = 0, 1
+MMC_QUIRK_NUDGE_CROSSING = 1
 (MMC_EV_BIRTH, MMC_EV_SCATTER, MMC_EV_CAPTURE, MMC_EV_FISSION,
  MMC_EV_SURFACE_CROSS, MMC_EV_LEAK, MMC_EV_VIRTUAL) = range(7)
 (MMC_EVAL_FIND_CELL, MMC_EVAL_NEAREST_SURFACE, MMC_EVAL_XS_TOTAL,
@@ -150,7 +151,8 @@ class Options(C.Structure):
     _fields_ = [("tracking", C.c_int32), ("tsl_mode", C.c_int32),
                 ("device", C.c_int32), ("block", C.c_int32),
                 ("max_in_flight", C.c_uint64), ("graph_iters", C.c_int32),
-                ("tail_threshold", C.c_int32), ("max_steps", C.c_uint64)]
+                ("tail_threshold", C.c_int32), ("max_steps", C.c_uint64),
+                ("quirks", C.c_int32), ("pad", C.c_int32)]
 
 
 class ParticleSoA(C.Structure):

@@ -107,6 +107,7 @@ struct DEst {
 
 struct DevWorld {
   int energy_mode, G, tracking, n_cells;
+  int quirks, pad_q;
   int n_est, n_est_current, n_est_flux, n_est_tsl;
   const DSurface* surfaces;
   const DCell* cells;
@@ -259,6 +260,31 @@ __device__ __forceinline__ int FindCell(const DevWorld& W, V3 p) {
   }
   return -1;
 }
+// Crossing without the nudge quirk: the crossed surface's sense is flipped
+// explicitly, every other face is tested on the point.
+__device__ __forceinline__ int FindCellAcross(const DevWorld& W, V3 p, int surf,
+                                              int new_inside) {
+  for (int c = 0; c < W.n_cells; c++) {
+    const DCell cell = W.cells[c];
+    bool in = true;
+    for (int k = 0; k < cell.face_count && in; k++) {
+      const DFace f = W.faces[cell.face_begin + k];
+      in = f.surface == surf ? f.inside == new_inside
+                             : SurfContains(W.surfaces[f.surface], p) == (f.inside != 0);
+    }
+    if (in) return c;
+  }
+  return -1;
+}
+__device__ __forceinline__ int FaceInside(const DevWorld& W, int cell, int surf) {
+  const DCell c = W.cells[cell];
+  for (int k = 0; k < c.face_count; k++) {
+    const DFace f = W.faces[c.face_begin + k];
+    if (f.surface == surf) return f.inside;
+  }
+  return 0;
+}
+
 // Cell::NearestSurface (Cell.cpp:37-51): first minimum wins.
 __device__ __forceinline__ double NearestSurface(const DevWorld& W, int cell, V3 p,
                                                  V3 d, int& surf) {

@@ -186,7 +186,7 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   R.B.surf[i] = -1;
   R.B.hist[i] = hist;
   R.B.steps[i] = 0;
-  ok = cell >= 0;
+  ok = cell >= 0 && W.cells[cell].material >= 0;  // born in void: leaks at once
   }
   uint32_t tot;
   const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
@@ -216,7 +216,7 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
   R.B.k_trk[i] = 0;
   R.B.k_abs[i] = 0;
   R.fis_cnt[i] = 0;
-  ok = cell >= 0;
+  ok = cell >= 0 && W.cells[cell].material >= 0;
   }
   uint32_t tot;
   const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
@@ -311,14 +311,17 @@ __global__ void __launch_bounds__(256) KCross(DevWorld W, RunState R, int iter, 
     if (valid) {
       p = R.QX[q];
       const V3 pos = LoadPos(R.B, p);
-      const int cell = FindCell(W, pos);
+      const int surf = R.B.surf[p];
+      const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
+                           ? FindCell(W, pos)
+                           : FindCellAcross(W, pos, surf, 1 - FaceInside(W, R.B.cell[p], surf));
       if (cell < 0) {
         atomicOr(R.misc, kErrLost);
       } else {
         R.B.cell[p] = cell;
         alive = W.cells[cell].material >= 0;  // surface_cross vs leak
         if (W.n_est_current)
-          ScoreCurrent(W, R, R.B.hist[p], R.B.surf[p], LoadDir(R.B, p), R.B.E[p]);
+          ScoreCurrent(W, R, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
       }
     }
     uint32_t tot;

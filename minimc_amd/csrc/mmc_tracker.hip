@@ -193,6 +193,7 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   W.energy_mode = wd->energy_mode;
   W.G = wd->groups;
   W.tracking = ctx->opt.tracking;
+  W.quirks = ctx->opt.quirks;
   W.n_cells = wd->n_cells;
   if (wd->n_cells <= 0 || wd->n_materials <= 0 || wd->n_nuclides <= 0)
     return Fail(ctx, MMC_ERR_ARG, "world needs cells, materials and nuclides");

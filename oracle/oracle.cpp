@@ -407,6 +407,7 @@ struct World {
   int energy_mode = MMC_MULTIGROUP;
   int G = 0;
   int tracking = MMC_SURFACE_TRACKING;
+  int quirks = 0;
   uint64_t max_steps = 1000000;
   std::vector<Surface> surfaces;
   std::vector<Field> fields;
@@ -439,6 +440,26 @@ int FindCell(const World& w, const Pt& p) {
     if (CellContains(w, w.cells[i], p)) return int(i);
   return -1;
 }
+// Crossing without the nudge quirk: the first cell whose face on the crossed
+// surface has the flipped sense and whose other faces contain the point.
+int FindCellAcross(const World& w, const Pt& p, int surf, bool new_inside) {
+  for (size_t i = 0; i < w.cells.size(); i++) {
+    bool ok = true;
+    for (const auto& f : w.cells[i].faces) {
+      ok = f.first == surf ? f.second == new_inside
+                           : Contains(w.surfaces[f.first], p) == f.second;
+      if (!ok) break;
+    }
+    if (ok) return int(i);
+  }
+  return -1;
+}
+bool FaceInside(const World& w, int cell, int surf) {
+  for (const auto& f : w.cells[cell].faces)
+    if (f.first == surf) return f.second;
+  return false;
+}
+
 // Cell::NearestSurface (Cell.cpp:37-51): min_element keeps the first of
 // equal distances; the winner's distance is computed again.
 std::pair<int, double> NearestSurface(const World& w, const Cell& c,
@@ -973,6 +994,10 @@ int Transport(Ctx& c, Particle& p) {
   World& w = *c.w;
   p.cell = FindCell(w, p.pos);
   if (p.cell < 0) return kErrLost;
+  if (w.cells[p.cell].material < 0) {  // born in a void cell: leaks at once
+    p.event = MMC_EV_LEAK;
+    return 0;
+  }
   uint64_t steps = 0;
   while (IsAlive(p)) {
     if (++steps > w.max_steps) return kErrSteps;
@@ -988,7 +1013,9 @@ int Transport(Ctx& c, Particle& p) {
       const double d = ns.second + kNudge;
       if (c.keig) c.acc->k_trk += d * m.aden * MaterialNuFission(w, p);
       p.pos = p.pos + p.dir * d;  // Particle::Stream (Particle.cpp:46-53)
-      p.cell = FindCell(w, p.pos);
+      p.cell = (w.quirks & MMC_QUIRK_NUDGE_CROSSING)
+                   ? FindCell(w, p.pos)
+                   : FindCellAcross(w, p.pos, ns.first, !FaceInside(w, p.cell, ns.first));
       if (p.cell < 0) return kErrLost;
       p.surface = ns.first;
       p.event = w.cells[p.cell].material >= 0 ? MMC_EV_SURFACE_CROSS
@@ -1125,6 +1152,7 @@ int orc_create(const mmc_world_desc* wd, const mmc_tally_desc* td,
   w.G = wd->groups;
   if (opt) {
     w.tracking = opt->tracking;
+    w.quirks = opt->quirks;
     if (opt->max_steps) w.max_steps = opt->max_steps;
   }
   for (int i = 0; i < wd->n_surfaces; i++) {
