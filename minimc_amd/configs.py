@@ -75,3 +75,54 @@ def point_absorber_sphere(radius=1.0, sigma=1.0):
     pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leakage", surface=s))
     pr.histories = 1000
     return pr
+
+
+# ---------------------------------------------------------------------------
+# Continuous-energy fixed-source slab.
+# One fictitious nuclide (awr 10) from synth.ce_nuclide (60,000 log-uniform
+# points, 1e-5 eV - 20 MeV; 1/v capture 0.5 b at 0.0253 eV; 4 b potential
+# scattering; 200 seeded Breit-Wigner resonances), 0.1 atoms/(b cm), slab
+# |x| < 20 cm, vacuum outside.  Isotropic point source at the origin, 2 MeV.
+# Leakage spectra on both faces, 300 log bins over 1e-11 - 20 MeV.
+C2_AWR = 10.0
+C2_ADEN = 0.1
+C2_HALF_WIDTH = 20.0
+
+
+def ce_tables(data, with_total=True, temperature=ROOM_TEMPERATURE, awr=C2_AWR):
+    E = data["E"]
+    n = dict(awr=awr, reactions=[
+        dict(kind=abi.MMC_RX_CAPTURE, xs=(E, data["capture"]), temperature=temperature),
+        dict(kind=abi.MMC_RX_SCATTER, xs=(E, data["elastic"]), temperature=temperature),
+    ])
+    if "fission" in data:
+        n["reactions"].append(dict(kind=abi.MMC_RX_FISSION, xs=(E, data["fission"]),
+                                   temperature=temperature, nubar=data["nubar"]))
+    if with_total:
+        n["total"] = (E, data["total"])
+        n["total_temperature"] = temperature
+    return n
+
+
+def c2_slab(histories=10_000_000, n_points=60000, n_res=200, half_width=C2_HALF_WIDTH,
+            aden=C2_ADEN):
+    from . import synth
+    pr = Problem(abi.MMC_CONTINUOUS)
+    pr.add_field()
+    s0 = pr.add_surface("left", abi.MMC_PLANEX, a=-half_width)
+    s1 = pr.add_surface("right", abi.MMC_PLANEX, a=half_width)
+    data = synth.ce_nuclide(n=n_points, n_res=n_res)
+    pr.nuclides.append(ce_tables(data))
+    m = pr.add_material("slab", aden, [(0, 1.0)])
+    pr.add_cell("slab", m, [(s0, False), (s1, True)])
+    pr.add_cell("", -1, [(s0, True)])
+    pr.add_cell("", -1, [(s1, False)])
+    pr.histories = histories
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_ISOTROPIC, E=2.0)
+    spec = dict(kind=abi.MMC_BINS_LOGSPACE, min=-11.0, max=float(np.log10(20.0)), bins=300)
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leak_left", surface=s0,
+                              energy=spec))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leak_right", surface=s1,
+                              energy=spec))
+    pr.data = data
+    return pr

@@ -1,0 +1,129 @@
+"""Oracle continuous-energy physics against closed forms.
+
+- ContinuousMap::at (ContinuousMap.hpp:20-56): values at grid points, lin-lin
+  between them (checked against numpy.interp), snapping outside.
+- free-gas cross-section adjustment (ContinuousReaction.cpp:225-237):
+  sigma(E, T) = sigma(E) F(T) / F(T_eval), F(a) = (1 + 1/2a) erf(sqrt a)
+  + exp(-a)/sqrt(pi a), a = A E / kT, checked with scipy's erf.
+- free-gas scattering kernel (ContinuousReaction.cpp:119-189): the energies
+  of successive collisions in a pure scatterer form a Markov chain whose
+  stationary law is the collision density phi(E) sigma_eff(E), with
+  phi ~ E exp(-E/kT) (detailed balance) and sigma_eff(E) ~ F(A E / kT), the
+  free-gas broadening above; for A = 1 its mean is 7/4 kT (quadrature).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf
+
+from minimc_amd import Problem, abi, configs, synth
+from oracle.oracle import Oracle, SoA
+
+K = 8.617333262145e-11
+
+
+def one_cell_problem(nuc, T=293.6, radius=1e6, aden=0.1):
+    pr = Problem(abi.MMC_CONTINUOUS)
+    pr.add_field(c=T)
+    s = pr.add_surface("s", abi.MMC_SPHERE, r=radius)
+    pr.nuclides.append(nuc)
+    m = pr.add_material("m", aden, [(0, 1.0)])
+    pr.add_cell("c", m, [(s, True)])
+    pr.add_cell("", -1, [(s, False)])
+    return pr
+
+
+@pytest.fixture(scope="module")
+def small_data():
+    return synth.ce_nuclide(n=2000, n_res=20)
+
+
+def test_table_interpolation(small_data):
+    d = small_data
+    o = Oracle(one_cell_problem(configs.ce_tables(d)))
+    E = d["E"]
+    rng = np.random.default_rng(0)
+    q = np.concatenate([E[::37], np.exp(rng.uniform(np.log(E[0]), np.log(E[-1]), 3000)),
+                        [E[0] * 0.5, E[-1] * 2.0]])
+    s = o.eval_event(abi.MMC_EVAL_XS_TOTAL, SoA(len(q), E=q, cell=0))
+    want = np.interp(q, E, d["total"])  # snaps to end values outside
+    np.testing.assert_allclose(s.out0, want, rtol=1e-13)
+    np.testing.assert_array_equal(s.out0[:len(E[::37])], d["total"][::37])
+
+
+def test_free_gas_cross_section_adjustment(small_data):
+    d = small_data
+    nuc = configs.ce_tables(d, with_total=False, awr=0.99)
+    T = 900.0
+    o = Oracle(one_cell_problem(nuc, T=T))
+    q = np.exp(np.linspace(np.log(1e-10), np.log(1e-5), 200))
+    s = o.eval_event(abi.MMC_EVAL_XS_TOTAL, SoA(len(q), E=q, cell=0))
+
+    def F(E, T):
+        a = 0.99 * E / (K * T)
+        return (1 + 1 / (2 * a)) * erf(np.sqrt(a)) + np.exp(-a) / np.sqrt(math.pi * a)
+    cap = np.interp(q, d["E"], d["capture"])
+    el = np.interp(q, d["E"], d["elastic"]) / F(q, 293.6) * F(q, T)
+    np.testing.assert_allclose(s.out0, cap + el, rtol=1e-12)
+    # majorant uses the field's upper bound (constant field: same T)
+    np.testing.assert_allclose(s.out1, s.out0, rtol=0)
+
+
+def test_evaluation_temperature_rule(small_data):
+    # (T_req - T_eval) / T_eval < 0.01 is "valid": colder cells use the table
+    # as is (quirk Q7, ContinuousEvaluation.cpp:17-20)
+    d = small_data
+    o = Oracle(one_cell_problem(configs.ce_tables(d, with_total=False, awr=5.0), T=100.0))
+    q = np.array([1e-9, 1e-7, 1e-3])
+    s = o.eval_event(abi.MMC_EVAL_XS_TOTAL, SoA(3, E=q, cell=0))
+    want = np.interp(q, d["E"], d["capture"]) + np.interp(q, d["E"], d["elastic"])
+    np.testing.assert_allclose(s.out0, want, rtol=1e-14)
+
+
+def test_free_gas_kernel_equilibrium():
+    E = synth.log_grid(200, 1e-11, 20.0)
+    nuc = dict(awr=1.0, reactions=[dict(kind=abi.MMC_RX_SCATTER,
+                                        xs=(E, np.full_like(E, 20.0)), temperature=293.6)])
+    T = 293.6
+    o = Oracle(one_cell_problem(nuc, T=T))
+    n = 20000
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    s = SoA(n, E=1e-6, u=d[:, 0], v=d[:, 1], w=d[:, 2], cell=0,
+            rng=rng.integers(1, 2**31 - 1, n))
+    means = []
+    for k in range(120):
+        s = o.eval_event(abi.MMC_EVAL_COLLIDE, s)
+        if k >= 60:
+            means.append(s.E.mean())
+    kT = K * T
+    m = np.mean(means)
+    from scipy.integrate import quad
+
+    def F(x):
+        return (1 + 1 / (2 * x)) * erf(np.sqrt(x)) + np.exp(-x) / np.sqrt(math.pi * x)
+
+    def w(x):
+        return x * np.exp(-x) * F(x)
+    want = quad(lambda x: x * w(x), 0, 60)[0] / quad(w, 0, 60)[0]   # 1.75
+    assert abs(m / kT - want) < 0.02, (m / kT, want)
+    assert abs(m / kT - 2.0) > 0.15  # sigma_eff weighting matters
+    assert np.all(s.event == abi.MMC_EV_SCATTER)
+
+
+def test_free_gas_stationary_target_limit():
+    # a heavy target at E >> kT: <E'> -> E (A^2 + 1) / (A + 1)^2
+    E = synth.log_grid(200, 1e-11, 20.0)
+    A = 10.0
+    nuc = dict(awr=A, reactions=[dict(kind=abi.MMC_RX_SCATTER,
+                                      xs=(E, np.full_like(E, 4.0)), temperature=293.6)])
+    o = Oracle(one_cell_problem(nuc))
+    n = 200000
+    rng = np.random.default_rng(2)
+    s = SoA(n, E=1.0, u=1.0, cell=0, rng=rng.integers(1, 2**31 - 1, n))
+    s = o.eval_event(abi.MMC_EVAL_COLLIDE, s)
+    want = (A * A + 1) / (A + 1) ** 2
+    se = s.E.std() / math.sqrt(n)
+    assert abs(s.E.mean() - want) < 4 * se + 1e-6
