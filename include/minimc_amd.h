@@ -403,6 +403,20 @@ int mmc_get_tallies(mmc_ctx* ctx, int estimator, double* S, double* S2,
 int mmc_reset_tallies(mmc_ctx* ctx);
 int mmc_get_counters(mmc_ctx* ctx, mmc_counters* out);
 
+/* Kernel timing: with profiling on, every event-kernel launch is bracketed
+ * by HIP events on the context's stream and its duration accumulated per
+ * kernel kind (mmc_kernel_kind).  Off by default. */
+enum mmc_kernel_kind {
+  MMC_K_ADVANCE = 0, /* XS lookup + collision distance + CSG distance */
+  MMC_K_CROSS = 1,   /* surface crossing + cell search + current tally */
+  MMC_K_COLLIDE = 2, /* nuclide + reaction sampling, collision physics */
+  MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling      */
+  MMC_K_N = 4
+};
+int mmc_set_profiling(mmc_ctx* ctx, int enable);
+/* total_ms[k] and launches[k] for k < MMC_K_N (accumulated since enable). */
+int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* launches);
+
 /* Single-event parity entry point: evaluates kind on n particle states
  * already in host memory (in place). */
 int mmc_eval_event(mmc_ctx* ctx, int kind, mmc_particle_soa* p, uint64_t n);

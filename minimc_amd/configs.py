@@ -81,9 +81,11 @@ def point_absorber_sphere(radius=1.0, sigma=1.0):
 # Continuous-energy fixed-source slab.
 # One fictitious nuclide (awr 10) from synth.ce_nuclide (60,000 log-uniform
 # points, 1e-5 eV - 20 MeV; 1/v capture 0.5 b at 0.0253 eV; 4 b potential
-# scattering; 200 seeded Breit-Wigner resonances), 0.1 atoms/(b cm), slab
-# |x| < 20 cm, vacuum outside.  Isotropic point source at the origin, 2 MeV.
-# Leakage spectra on both faces, 300 log bins over 1e-11 - 20 MeV.
+# scattering; 200 seeded Breit-Wigner resonances between 1 eV and 100 keV,
+# peaks 3-300 b, capture share 0.1-1 %), 0.1 atoms/(b cm), slab |x| < 20 cm,
+# vacuum outside.  Isotropic point source at the origin, 2 MeV.  About 70
+# collisions per history, a quarter of the histories leak.  Leakage spectra
+# on both faces, 300 log bins over 1e-11 - 20 MeV.
 C2_AWR = 10.0
 C2_ADEN = 0.1
 C2_HALF_WIDTH = 20.0

@@ -67,6 +67,13 @@ struct mmc_ctx {
   unsigned long long ev_host[kEvN] = {};
   // surrogate
   MlpState mlp{};
+  // kernel timing
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kind;  // kind of the launch bracketed by pair i
+  size_t ev_used = 0;
+  double k_ms[MMC_K_N] = {};
+  uint64_t k_launches[MMC_K_N] = {};
 };
 
 template <class T>
@@ -548,6 +555,7 @@ extern "C" void mmc_destroy(mmc_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void* p : ctx->allocs) hipFree(p);
   MlpFree(ctx->mlp);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -678,16 +686,54 @@ static int CheckMisc(mmc_ctx* ctx, const uint32_t* misc) {
   return Fail(ctx, MMC_ERR_ARG, "alpha sampling outside the beta grid");
 }
 
+// Kernel timing brackets (profiling mode only).
+static void ProfBegin(mmc_ctx* ctx, int kind) {
+  if (!ctx->prof) return;
+  if (ctx->ev_used + 2 > ctx->ev_pool.size()) {
+    for (int k = 0; k < 64; k++) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      ctx->ev_pool.push_back(e);
+    }
+  }
+  hipEventRecord(ctx->ev_pool[ctx->ev_used], ctx->stream);
+  ctx->ev_kind.push_back(kind);
+}
+static void ProfEnd(mmc_ctx* ctx) {
+  if (!ctx->prof) return;
+  hipEventRecord(ctx->ev_pool[ctx->ev_used + 1], ctx->stream);
+  ctx->ev_used += 2;
+}
+// After a stream synchronisation: fold the recorded pairs into the totals.
+static void ProfCollect(mmc_ctx* ctx) {
+  if (!ctx->prof) return;
+  for (size_t i = 0; i < ctx->ev_kind.size(); i++) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev_pool[2 * i], ctx->ev_pool[2 * i + 1]);
+    ctx->k_ms[ctx->ev_kind[i]] += ms;
+    ctx->k_launches[ctx->ev_kind[i]]++;
+  }
+  ctx->ev_kind.clear();
+  ctx->ev_used = 0;
+}
+
 static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
   const size_t lds = ctx->W.energy_mode == 1 && ctx->n_hash0 > 0
                          ? sizeof(int) * (ctx->n_hash0 + 1) : 0;
+  ProfBegin(ctx, MMC_K_ADVANCE);
   hipLaunchKernelGGL(KAdvance, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
                      iter, qin);
+  ProfEnd(ctx);
+  ProfBegin(ctx, MMC_K_CROSS);
   hipLaunchKernelGGL(KCross, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R, iter,
                      qin);
+  ProfEnd(ctx);
+  ProfBegin(ctx, MMC_K_COLLIDE);
   hipLaunchKernelGGL(KCollide, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
                      iter, qin);
+  ProfEnd(ctx);
   if (ctx->has_tsl) {
+    ProfBegin(ctx, MMC_K_TSL);
     if (ctx->opt.tsl_mode == MMC_TSL_SURROGATE && ctx->mlp.ready) {
       const int rc = MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, grid, ctx->stream);
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
@@ -695,6 +741,29 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
       hipLaunchKernelGGL(KTslPod, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                          iter, qin);
     }
+    ProfEnd(ctx);
+  }
+  return MMC_OK;
+}
+
+extern "C" int mmc_set_profiling(mmc_ctx* ctx, int enable) {
+  if (!ctx) return MMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ProfCollect(ctx);
+  ctx->prof = enable != 0;
+  for (int k = 0; k < MMC_K_N; k++) {
+    ctx->k_ms[k] = 0;
+    ctx->k_launches[k] = 0;
+  }
+  return MMC_OK;
+}
+extern "C" int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* launches) {
+  if (!ctx || !total_ms || !launches) return MMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ProfCollect(ctx);
+  for (int k = 0; k < MMC_K_N; k++) {
+    total_ms[k] = ctx->k_ms[k];
+    launches[k] = ctx->k_launches[k];
   }
   return MMC_OK;
 }
@@ -721,6 +790,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ProfCollect(ctx);
     ctx->cnt.iterations += K;
     const int rc = CheckMisc(ctx, misc);
     if (rc) return rc;

@@ -20,7 +20,8 @@ def log_grid(n=60000, e_min=1e-5 * EV, e_max=20.0):
 
 
 def ce_nuclide(n=60000, n_res=200, seed=20261202, sigma_c0=0.5, sigma_pot=4.0,
-               res_lo=1.0 * EV, res_hi=1e5 * EV, e_min=1e-5 * EV, e_max=20.0):
+               res_lo=1.0 * EV, res_hi=1e5 * EV, e_min=1e-5 * EV, e_max=20.0,
+               log_peak=(0.5, 2.5), capture_share=(0.001, 0.01)):
     """Returns dict(E, capture, elastic, total) on one shared grid.
 
     capture = sigma_c0 * sqrt(0.0253 eV / E) + sum_r capture resonances
@@ -33,8 +34,8 @@ def ce_nuclide(n=60000, n_res=200, seed=20261202, sigma_c0=0.5, sigma_pot=4.0,
     E = log_grid(n, e_min, e_max)
     e_r = np.exp(rng.uniform(np.log(res_lo), np.log(res_hi), n_res))
     gamma = e_r * 10 ** rng.uniform(-3.0, -1.5, n_res)   # relative widths
-    sigma0 = 10 ** rng.uniform(1.5, 4.0, n_res)          # barns at peak
-    g = rng.uniform(0.1, 0.9, n_res)                     # capture share
+    sigma0 = 10 ** rng.uniform(*log_peak, n_res)          # barns at peak
+    g = rng.uniform(*capture_share, n_res)                # capture share
     capture = sigma_c0 * np.sqrt(0.0253 * EV / E)
     elastic = np.full_like(E, sigma_pot)
     for k in range(n_res):

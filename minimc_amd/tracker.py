@@ -38,6 +38,8 @@ _SYMBOLS = {
                          C.c_uint64], C.c_int),
     "mmc_reset_tallies": ([C.c_void_p], C.c_int),
     "mmc_get_counters": ([C.c_void_p, C.POINTER(abi.Counters)], C.c_int),
+    "mmc_set_profiling": ([C.c_void_p, C.c_int], C.c_int),
+    "mmc_get_kernel_times": ([C.c_void_p, abi.c_double_p, C.POINTER(C.c_uint64)], C.c_int),
     "mmc_eval_event": ([C.c_void_p, C.c_int, C.POINTER(abi.ParticleSoA), C.c_uint64],
                        C.c_int),
     "mmc_sample_source": ([C.c_void_p, C.POINTER(abi.SourceDesc), C.c_uint64, C.c_uint64,
@@ -172,6 +174,17 @@ class Tracker:
         c = abi.Counters()
         self._check(lib().mmc_get_counters(self.h, C.byref(c)))
         return c.as_dict()
+
+    def set_profiling(self, on=True):
+        self._check(lib().mmc_set_profiling(self.h, 1 if on else 0))
+
+    KERNELS = ("advance", "cross", "collide", "tsl")
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        n = (C.c_uint64 * 4)()
+        self._check(lib().mmc_get_kernel_times(self.h, abi.dptr(ms), n))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNELS)}
 
     # -- single events ------------------------------------------------------
     def eval_event(self, kind, soa):
