@@ -465,12 +465,17 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t
   const bool snap_lower = (sgn == 1 && t.beta_cutoff <= betas[b_hi]) ||
                           (sgn == -1 && -betas[b_hi] < -E / (kBoltzmann * T));
   const bool snap_min = b_hi == 0;
-  if (snap_lower && b_hi == 0) return kErrAlphaRange;
   const double r = snap_lower ? 0
                    : snap_min ? 1
                               : (abs_b - betas[b_hi - 1] / (betas[b_hi] - betas[b_hi - 1]));
-  const int b_s_i = r <= Canonical(g) ? b_hi - 1 : b_hi;
-  const double b_s = sgn * betas[b_s_i];
+  // |b| below the first grid beta with the lower-snap rule: the reference
+  // reads betas.at(-1) (ThermalScattering.cpp:366-368); use the first beta.
+  int b_s_i = r <= Canonical(g) ? b_hi - 1 : b_hi;
+  if (b_hi == 0) b_s_i = 0;
+  double b_s = sgn * betas[b_s_i];
+  // ... and if that grid beta would make E' negative, the kinematic bounds
+  // are those of the sampled beta itself.
+  if (E + b_s * kBoltzmann * T < 0) b_s = b;
   const double sqrt_E = sqrt(E);
   const double bs_root = sqrt(E + b_s * kBoltzmann * T);
   const double akT = t.awr * kBoltzmann * T;

@@ -751,13 +751,19 @@ int SampleAlpha(const Tsl& t, Lcg& g, double b, double E, double T,
       (sgn == 1 && t.beta_cutoff <= t.betas[b_hi]) ||
       (sgn == -1 && -t.betas[b_hi] < -E / (kBoltzmann * T));
   const bool snap_min = b_hi == 0;
-  if (snap_lower && b_hi == 0) return kErrAlphaRange;
   const double r =
       snap_lower ? 0
       : snap_min ? 1
                  : (abs_b - t.betas[b_hi - 1] / (t.betas[b_hi] - t.betas[b_hi - 1]));
-  const size_t b_s_i = r <= Canonical(g) ? b_hi - 1 : b_hi;
-  const double b_s = sgn * t.betas[b_s_i];
+  // With |b| below the first grid beta and the lower-snap rule firing, the
+  // reference reads betas.at(-1) and throws (ThermalScattering.cpp:366-368);
+  // the first grid beta is used instead.
+  size_t b_s_i = r <= Canonical(g) ? b_hi - 1 : b_hi;
+  if (b_hi == 0) b_s_i = 0;
+  double b_s = sgn * t.betas[b_s_i];
+  // ... and if that grid beta would make E' negative, the kinematic bounds
+  // are those of the sampled beta itself.
+  if (E + b_s * kBoltzmann * T < 0) b_s = b;
   const double sqrt_E = std::sqrt(E);
   const double bs_root = std::sqrt(E + b_s * kBoltzmann * T);
   const double akT = t.awr * kBoltzmann * T;
@@ -1503,6 +1509,23 @@ int orc_run_fixed_source(orc_world* ow, const mmc_source_desc* src,
   for (auto& th : pool) th.join();
   for (auto& a : accs) Merge(w.total, a);
   return ErrToStatus(ow, w.total.err);
+}
+
+int orc_tsl_evaluate(orc_world* ow, int tsl, int which, int part, uint64_t cdf_i,
+                     uint64_t x_i, double T, double* out) {
+  if (!ow || !out || tsl < 0 || tsl >= int(ow->w.tsl.size())) return MMC_ERR_ARG;
+  const Tsl& t = ow->w.tsl[tsl];
+  const auto& parts = which == 0 ? t.beta_parts : t.alpha_parts;
+  if (part < 0 || part >= int(parts.size())) return MMC_ERR_ARG;
+  const TslPartition& P = parts[part];
+  if (cdf_i >= P.F.size() || x_i >= P.x.size()) return MMC_ERR_ARG;
+  *out = P.Evaluate(cdf_i, x_i, T);
+  return 0;
+}
+int orc_tsl_total(orc_world* ow, int tsl, double E, double T, double* out) {
+  if (!ow || !out || tsl < 0 || tsl >= int(ow->w.tsl.size())) return MMC_ERR_ARG;
+  *out = TslTotal(ow->w.tsl[tsl], E, T);
+  return 0;
 }
 
 int orc_set_entropy_mesh(orc_world* ow, const int32_t n[3], const double lo[3],

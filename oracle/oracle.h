@@ -52,6 +52,12 @@ int orc_get_counters(orc_world* w, mmc_counters* out);
 int orc_tsl_sample(orc_world* w, int tsl, uint64_t n, const double* E,
                    const double* T, uint32_t* rng, double* alpha,
                    double* beta, int threads);
+/* POD reconstruction of one partition entry (BetaPartition/AlphaPartition::
+ * Evaluate): which = 0 beta partitions, 1 alpha partitions. */
+int orc_tsl_evaluate(orc_world* w, int tsl, int which, int part, uint64_t cdf_i,
+                     uint64_t x_i, double T, double* out);
+/* ThermalScattering::GetTotal at (E, T). */
+int orc_tsl_total(orc_world* w, int tsl, double E, double T, double* out);
 int orc_set_entropy_mesh(orc_world* w, const int32_t n[3], const double lo[3],
                          const double hi[3]);
 

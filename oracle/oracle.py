@@ -54,6 +54,9 @@ def lib():
                                      abi.c_double_p, C.c_int]
         L.orc_set_entropy_mesh.argtypes = [vp, C.POINTER(C.c_int32),
                                            abi.c_double_p, abi.c_double_p]
+        L.orc_tsl_evaluate.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                       C.c_uint64, C.c_double, abi.c_double_p]
+        L.orc_tsl_total.argtypes = [vp, C.c_int, C.c_double, C.c_double, abi.c_double_p]
         L.orc_rng_draws.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
         L.orc_rng_canonicals.argtypes = [C.c_uint64, C.c_uint64, abi.c_double_p]
         L.orc_rng_exponentials.argtypes = [C.c_uint64, C.c_double, C.c_uint64,
@@ -205,6 +208,17 @@ class Oracle:
         c = abi.Counters()
         self._check(lib().orc_get_counters(self.h, C.byref(c)))
         return c.as_dict()
+
+    def tsl_evaluate(self, which, part, cdf_i, x_i, T, tsl=0):
+        out = C.c_double()
+        self._check(lib().orc_tsl_evaluate(self.h, tsl, which, part, cdf_i, x_i, T,
+                                           C.byref(out)))
+        return out.value
+
+    def tsl_total(self, E, T, tsl=0):
+        out = C.c_double()
+        self._check(lib().orc_tsl_total(self.h, tsl, E, T, C.byref(out)))
+        return out.value
 
     def tsl_sample(self, E, T, rng, tsl=0, threads=1):
         E = np.ascontiguousarray(E, dtype=np.float64)

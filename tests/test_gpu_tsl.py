@@ -1,0 +1,95 @@
+"""GPU thermal scattering (POD) vs the oracle on identical states."""
+import numpy as np
+import pytest
+
+from minimc_amd import abi, configs
+from oracle.oracle import Oracle, SoA
+from tests.gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+K = 8.617333262145e-11
+
+
+@pytest.fixture(scope="module")
+def pair():
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = configs.tsl_box(configs.tsl_tables(), T=600.0)
+    return pr, Oracle(pr), Tracker(pr, max_in_flight=1 << 16)
+
+
+def _states(n, seed):
+    rng = np.random.default_rng(seed)
+    E = np.exp(rng.uniform(np.log(1e-11), np.log(3.99e-6), n))
+    T = rng.uniform(300, 1200, n)
+    return SoA(n, E=E, out0=T, cell=0, u=1.0, rng=rng.integers(1, 2**31 - 1, n))
+
+
+def test_tsl_sample_event_parity(pair):
+    pr, o, t = pair
+    s = _states(100000, 0)
+    a = o.eval_event(abi.MMC_EVAL_TSL_SAMPLE, s.copy())
+    b = t.eval_event(abi.MMC_EVAL_TSL_SAMPLE, s.copy())
+    same = a.rng == b.rng
+    assert same.mean() > 0.999
+    for x, y in ((a.out0, b.out0), (a.out1, b.out1)):
+        x, y = x[same], y[same]
+        assert np.all(np.abs(x - y) <= 1e-11 * np.maximum(np.abs(x), 1e-6)), \
+            np.max(np.abs(x - y) / np.maximum(np.abs(x), 1e-6))
+
+
+def test_tsl_batch_sampler_matches_event(pair):
+    import torch
+    pr, o, t = pair
+    s = _states(200000, 1)
+    dev = torch.device("cuda:0")
+    E = torch.tensor(s.E, device=dev)
+    T = torch.tensor(s.out0, device=dev)
+    rng = torch.tensor(s.rng.astype(np.int32), device=dev)
+    alpha = torch.empty_like(E)
+    beta = torch.empty_like(E)
+    torch.cuda.synchronize()
+    t.tsl_sample_pod(E, T, rng, alpha, beta)
+    a, b, r = o.tsl_sample(s.E, s.out0, s.rng, threads=8)
+    same = r == rng.cpu().numpy().astype(np.uint32)
+    assert same.mean() > 0.999
+    np.testing.assert_allclose(alpha.cpu().numpy()[same], a[same], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(beta.cpu().numpy()[same], b[same], rtol=1e-11, atol=1e-12)
+
+
+def test_tsl_total_and_collide(pair):
+    pr, o, t = pair
+    n = 50000
+    rng = np.random.default_rng(2)
+    E = np.exp(rng.uniform(np.log(1e-11), np.log(20.0), n))
+    s = SoA(n, E=E, cell=0, rng=rng.integers(1, 2**31 - 1, n), u=1.0)
+    a = o.eval_event(abi.MMC_EVAL_XS_TOTAL, s.copy())
+    b = t.eval_event(abi.MMC_EVAL_XS_TOTAL, s.copy())
+    np.testing.assert_allclose(b.out0, a.out0, rtol=1e-13)
+    np.testing.assert_allclose(b.out1, a.out1, rtol=1e-13)
+    a = o.eval_event(abi.MMC_EVAL_COLLIDE, s.copy())
+    b = t.eval_event(abi.MMC_EVAL_COLLIDE, s.copy())
+    same = a.rng == b.rng
+    assert same.mean() > 0.999
+    assert np.array_equal(a.event[same], b.event[same])
+    np.testing.assert_allclose(b.E[same], a.E[same], rtol=1e-11)
+
+
+def test_broomstick_histograms_match_oracle():
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = configs.c3_broomstick()
+    n = 50000
+    o = Oracle(pr)
+    o.run_fixed_source(n, threads=8)
+    t = Tracker(pr, max_in_flight=1 << 14)
+    t.run_fixed_source(n)
+    for e in range(2):   # alpha, beta histograms
+        So, _ = o.tallies(e)
+        St, _ = t.tallies(e)
+        assert abs(So.sum() - St.sum()) <= 0.01 * So.sum() + 5
+        go = So.reshape(-1, 2)[:100].sum(1) if len(So) % 2 == 0 else So
+        gt = St.reshape(-1, 2)[:100].sum(1) if len(St) % 2 == 0 else St
+        assert np.all(np.abs(go - gt) <= 4 * np.sqrt(go + gt + 1) + 3)
+    c = t.counters()
+    assert c["tsl_collisions"] > 0.5 * n
