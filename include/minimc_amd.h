@@ -426,6 +426,13 @@ int mmc_eval_event(mmc_ctx* ctx, int kind, mmc_particle_soa* p, uint64_t n);
 int mmc_sample_source(mmc_ctx* ctx, const mmc_source_desc* source,
                       uint64_t seed0, uint64_t n, mmc_particle_soa* out);
 
+/* ---- device memory (for callers without a HIP runtime of their own) ------
+ * Allocations are on the context's device; copies are synchronous. */
+int mmc_device_malloc(mmc_ctx* ctx, uint64_t bytes, void** ptr);
+int mmc_device_free(mmc_ctx* ctx, void* ptr);
+int mmc_memcpy_h2d(mmc_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int mmc_memcpy_d2h(mmc_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
 /* ---- thermal-scattering samplers on device-resident batches -------------
  * Pointers are device pointers.  E in MeV, T in K, rng minstd states
  * (updated in place).  alpha, beta outputs.  tsl selects the table. */

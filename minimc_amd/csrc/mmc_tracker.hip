@@ -1374,6 +1374,31 @@ extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const doubl
   return CheckMisc(ctx, misc);
 }
 
+extern "C" int mmc_device_malloc(mmc_ctx* ctx, uint64_t bytes, void** ptr) {
+  if (!ctx || !ptr) return MMC_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  return MMC_OK;
+}
+extern "C" int mmc_device_free(mmc_ctx* ctx, void* ptr) {
+  if (!ctx) return MMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(ptr));
+  return MMC_OK;
+}
+extern "C" int mmc_memcpy_h2d(mmc_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return MMC_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MMC_OK;
+}
+extern "C" int mmc_memcpy_d2h(mmc_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return MMC_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MMC_OK;
+}
+
 hipStream_t MmcStream(mmc_ctx* ctx) { return ctx->stream; }
 MlpState& MmcMlp(mmc_ctx* ctx) { return ctx->mlp; }
 const DevWorld& MmcWorld(mmc_ctx* ctx) { return ctx->W; }

@@ -44,6 +44,10 @@ _SYMBOLS = {
                        C.c_int),
     "mmc_sample_source": ([C.c_void_p, C.POINTER(abi.SourceDesc), C.c_uint64, C.c_uint64,
                            C.POINTER(abi.ParticleSoA)], C.c_int),
+    "mmc_device_malloc": ([C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "mmc_device_free": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mmc_memcpy_h2d": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
+    "mmc_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "mmc_tsl_sample_pod": ([C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p,
                             C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mmc_mlp_set": ([C.c_void_p, C.c_int, C.POINTER(abi.MlpDesc), abi.c_float_p,
@@ -81,6 +85,54 @@ def device_count():
     n = C.c_int(0)
     lib().mmc_device_count(C.byref(n))
     return n.value
+
+
+class DeviceArray:
+    """A device buffer owned by a Tracker's context (the library's own HIP
+    runtime; no second runtime is loaded into the process)."""
+
+    def __init__(self, tracker, shape, dtype):
+        self.t = tracker
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        tracker._check(lib().mmc_device_malloc(tracker.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, tracker, a):
+        a = np.ascontiguousarray(a)
+        d = cls(tracker, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        self.t._check(lib().mmc_memcpy_h2d(self.t.h, self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.t._check(lib().mmc_memcpy_d2h(self.t.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
+    def data_ptr(self):
+        return self.ptr
+
+    def free(self):
+        if self.ptr and self.t.h:
+            lib().mmc_device_free(self.t.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class MmcError(RuntimeError):
@@ -198,7 +250,12 @@ class Tracker:
         self._check(lib().mmc_sample_source(self.h, C.byref(sd), seed0, n, C.byref(s)))
         return soa
 
-    # -- device batches (torch tensors on the device) ------------------------
+    # -- device batches (DeviceArray or any object with numel/data_ptr) -------
+    def device_array(self, a=None, shape=None, dtype=None):
+        if a is not None:
+            return DeviceArray.from_numpy(self, a)
+        return DeviceArray(self, shape, dtype)
+
     def tsl_sample_pod(self, E, T, rng, alpha, beta, tsl=0):
         self._check(lib().mmc_tsl_sample_pod(
             self.h, tsl, E.numel(), E.data_ptr(), T.data_ptr(), rng.data_ptr(),

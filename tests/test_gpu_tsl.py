@@ -39,22 +39,19 @@ def test_tsl_sample_event_parity(pair):
 
 
 def test_tsl_batch_sampler_matches_event(pair):
-    import torch
     pr, o, t = pair
     s = _states(200000, 1)
-    dev = torch.device("cuda:0")
-    E = torch.tensor(s.E, device=dev)
-    T = torch.tensor(s.out0, device=dev)
-    rng = torch.tensor(s.rng.astype(np.int32), device=dev)
-    alpha = torch.empty_like(E)
-    beta = torch.empty_like(E)
-    torch.cuda.synchronize()
+    E = t.device_array(s.E)
+    T = t.device_array(s.out0)
+    rng = t.device_array(s.rng)
+    alpha = t.device_array(shape=s.n, dtype=np.float64)
+    beta = t.device_array(shape=s.n, dtype=np.float64)
     t.tsl_sample_pod(E, T, rng, alpha, beta)
     a, b, r = o.tsl_sample(s.E, s.out0, s.rng, threads=8)
-    same = r == rng.cpu().numpy().astype(np.uint32)
+    same = r == rng.numpy()
     assert same.mean() > 0.999
-    np.testing.assert_allclose(alpha.cpu().numpy()[same], a[same], rtol=1e-11, atol=1e-15)
-    np.testing.assert_allclose(beta.cpu().numpy()[same], b[same], rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(alpha.numpy()[same], a[same], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(beta.numpy()[same], b[same], rtol=1e-11, atol=1e-12)
 
 
 def test_tsl_total_and_collide(pair):
