@@ -457,9 +457,18 @@ typedef struct mmc_mlp_desc {
 int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* desc,
                 const float* params, uint64_t n_params);
 int mmc_mlp_get(mmc_ctx* ctx, float* params, uint64_t n_params);
-/* Device batch inference: x [n x 4] (raw log E, T, u1, u2) -> y [n x 2]
- * (alpha, beta); FP32 in and out, BF16 MFMA inside. */
+/* Device batch inference of the network: x [n x 4] raw inputs (ln E[MeV],
+ * T[K], u1, u2) -> y [n x 2] raw outputs (the scaled energy transfer
+ * beta kT / (E + kT) and the position of alpha inside its kinematic range
+ * [alpha_min(beta), alpha_max(beta)]); FP32 in and out, BF16 MFMA with
+ * FP32 accumulation inside (widths 4-256-256-256-2 only). */
 int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y);
+/* The deterministic POD transport map the network is trained to imitate:
+ * the POD inverse CDFs of beta (level u1) and then alpha (level u2), with
+ * linear instead of stochastic interpolation between the tabulated
+ * energies and betas.  Device x [n x 4] raw inputs -> y [n x 2] raw
+ * outputs in the network's output parametrisation. */
+int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y);
 /* Surrogate sampling of (alpha, beta) for device batches with fallback. */
 int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n,
                              const double* E, const double* T, uint32_t* rng,
@@ -477,6 +486,10 @@ typedef struct mmc_train_opts {
 } mmc_train_opts;
 int mmc_mlp_train(mmc_ctx* ctx, uint64_t n, const float* x, const float* y,
                   const mmc_train_opts* opts, float* last_loss);
+/* Normalises raw device training data in place with the network's shifts
+ * and scales: x <- (x - in_shift) * in_scale, y <- (y - out_shift) /
+ * out_scale (y may be NULL). */
+int mmc_mlp_normalize(mmc_ctx* ctx, uint64_t n, float* x, float* y);
 /* One forward+backward on a host batch (normalised units): returns the
  * loss L = sum over rows and outputs of (f(x) - y)^2 / (2 n n_out) and, if
  * grads is not NULL, dL/dparams in the params layout.  For gradient checks

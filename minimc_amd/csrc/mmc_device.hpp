@@ -452,6 +452,39 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uin
   }
   return false;
 }
+// find_cdf of SampleAlpha (ThermalScattering.cpp:395-418): upper_bound over
+// the CDF axis with a POD evaluation inside the comparator.
+__device__ __forceinline__ double FindCdf(const DevWorld& W, const DTslPart& P, int local,
+                                          double T, double a, double alpha_cutoff) {
+  const double* F = W.pool + P.F_off;
+  const int nF = P.nF;
+  int first = 0, len = nF;
+  while (len > 0) {
+    const int half = len >> 1;
+    const int mid = first + half;
+    if (a < PartEvaluate(W, P, mid, local, T)) {
+      len = half;
+    } else {
+      first = mid + 1;
+      len = len - half - 1;
+    }
+  }
+  const int hi = first;
+  const double F_lo = hi != 0 ? F[hi - 1] : 0;
+  const double F_hi = hi != nF ? F[hi] : 1;
+  const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
+  const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : alpha_cutoff;
+  return F_lo + (a - a_lo) * (F_hi - F_lo) / (a_hi - a_lo);
+}
+// Partition of a concatenated grid index (beta_partition_E_ends /
+// alpha_partition_beta_ends) and the index local to it.
+__device__ __forceinline__ int PartitionOf(const int* ends, int n_parts, int idx, int& local) {
+  int p_i = 0;
+  while (p_i < n_parts && ends[p_i] <= idx) p_i++;
+  local = p_i == 0 ? idx : idx - ends[p_i - 1];
+  return p_i;
+}
+
 // ThermalScattering::SampleAlpha (ThermalScattering.cpp:340-463), quirk Q2
 // (operator precedence at :361-365) reproduced.
 __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t,
@@ -489,30 +522,10 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t
   const int local = p_i == 0 ? b_s_i : b_s_i - ends[p_i - 1];
   const double* F = W.pool + P.F_off;
   const int nF = P.nF;
-  double Fm[2];
-  const double targets[2] = {a_min, a_max};
-  for (int q = 0; q < 2; q++) {  // find_cdf (:395-418)
-    const double a = targets[q];
-    int first = 0, len = nF;
-    while (len > 0) {
-      const int half = len >> 1;
-      const int mid = first + half;
-      if (a < PartEvaluate(W, P, mid, local, T)) {
-        len = half;
-      } else {
-        first = mid + 1;
-        len = len - half - 1;
-      }
-    }
-    const int hi = first;
-    const double F_lo = hi != 0 ? F[hi - 1] : 0;
-    const double F_hi = hi != nF ? F[hi] : 1;
-    const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
-    const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
-    Fm[q] = F_lo + (a - a_lo) * (F_hi - F_lo) / (a_hi - a_lo);
-  }
+  const double F_min = FindCdf(W, P, local, T, a_min, t.alpha_cutoff);
+  const double F_max = FindCdf(W, P, local, T, a_max, t.alpha_cutoff);
   for (int k = 0; k < kAlphaResampleLimit; k++) {
-    const double u = Fm[0] + Canonical(g) * (Fm[1] - Fm[0]);
+    const double u = F_min + Canonical(g) * (F_max - F_min);
     const int hi = UpperBoundArr(F, nF, u);
     const double F_lo = hi != 0 ? F[hi - 1] : 0;
     const double F_hi = hi != nF ? F[hi] : 1;

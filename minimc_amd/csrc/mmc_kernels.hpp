@@ -77,7 +77,7 @@ __device__ __forceinline__ void WaveCount(unsigned long long* c, bool pred) {
   const uint64_t mask = __ballot(pred);
   if (__lane_id() == 0 && mask) atomicAdd(c, (unsigned long long)__popcll(mask));
 }
-__device__ __forceinline__ uint32_t* Ring(const RunState& R, int iter, int slot) {
+__host__ __device__ __forceinline__ uint32_t* Ring(const RunState& R, int iter, int slot) {
   return R.ring + ((iter + kRing) % kRing) * kRingStride + slot;
 }
 

@@ -1,31 +1,873 @@
-// Thermal-scattering surrogate MLP (placeholder until the kernels land).
+// Thermal-scattering surrogate: an MLP (ln E, T, u1, u2) -> (y1, y2) that
+// replaces the POD inverse-CDF sampling of ThermalScattering::Scatter
+// (ThermalScattering.cpp:159-171, :271-463).  y1 = beta kT / (E + kT) and
+// y2 = (alpha - alpha_min(beta)) / (alpha_max(beta) - alpha_min(beta)).
+//
+// - POD transport map (training targets): the same POD tables and
+//   inverse-CDF arithmetic as SampleBeta/SampleAlpha, with linear
+//   interpolation between grid energies / betas instead of a random pick.
+// - Training: FP32 tiled GEMMs on the vector ALUs, SiLU, MSE loss, Adam.
+// - Inference: one fused kernel per batch.  Layer 0 (4 -> 256) and layer 3
+//   (256 -> 2) run on the vector ALUs; the two 256 x 256 layers run on
+//   v_mfma_f32_32x32x16_bf16 with the weights staged in LDS.  Each layer is
+//   computed transposed (Z^T = W H^T) so the accumulator layout of one layer
+//   is the B-operand layout of the next once the next layer's input
+//   features are permuted; activations never leave registers.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "../../include/minimc_amd.h"
 #include "mmc_mlp.hpp"
 
+using namespace mmc;
+
+hipStream_t MmcStream(mmc_ctx* ctx);
+MlpState& MmcMlp(mmc_ctx* ctx);
+const DevWorld& MmcWorld(mmc_ctx* ctx);
+int MmcFail(mmc_ctx* ctx, int code, const char* msg);
+int MmcDevice(mmc_ctx* ctx);
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
 namespace mmc {
-void MlpFree(MlpState& m) { m.ready = false; }
-int MlpTslStep(MlpState&, const DevWorld&, const RunState&, int, int, uint32_t, hipStream_t) {
-  return 1;
+
+constexpr int kHid = 256;           // hidden width of the fused kernel
+constexpr int kFragElems = 8 * 16 * 64 * 8;  // bf16 per 256x256 layer (65536)
+constexpr int kInferWaves = 4;
+
+// Device view of the network for the fused kernel.
+struct MlpDev {
+  const bf16x8* frag1;  // layer 1 A fragments [t 8][kb 16][lane 64]
+  const bf16x8* frag2;  // layer 2
+  const float* p0;      // layer 0: [h 2][kb 16][q 8][5] = w0[f][0..3], b0[f]
+  const float* bias1;   // [t 8][r 16][h 2]
+  const float* bias2;
+  const float* p3;      // layer 3: [h 2][t 8][r 16][2]
+  float b3[2];
+  float in_shift[4], in_scale[4], out_shift[2], out_scale[2];
+};
+
+// Input feature of the K-permuted B fragment (K block kb, lane half h,
+// element q); equal to the accumulator feature of tile kb/2, register
+// 8 (kb % 2) + q, lane half h.
+__host__ __device__ __forceinline__ int FeatB(int kb, int h, int q) {
+  return 32 * (kb / 2) + (2 * (kb % 2) + q / 4) * 8 + 4 * h + q % 4;
 }
+// Output feature of accumulator register r of tile t in lane half h.
+__host__ __device__ __forceinline__ int FeatC(int t, int r, int h) {
+  return 32 * t + (r / 4) * 8 + 4 * h + r % 4;
+}
+
+__device__ __forceinline__ float Silu(float z) { return z / (1.0f + __expf(-z)); }
+
+void MlpFree(MlpState& m) {
+  void* ps[] = {m.params, m.wbf16, m.adam_m, m.adam_v, m.grads, m.act, m.loss,
+                m.xin, m.yout, m.fb, m.packed};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  m = MlpState{};
+}
+
+// ---------------------------------------------------------------------------
+// Fused BF16 MFMA inference.  Block = kInferWaves waves, each wave owns 32
+// rows per pass; the loop is block-uniform (barriers inside).  `n_ptr`, when
+// set, points at a device row count (the tracker's TSL queue length).
+__global__ void __launch_bounds__(64 * kInferWaves)
+    KMlpFused(MlpDev m, const float* __restrict__ x, float* __restrict__ y,
+              const uint32_t* n_ptr, uint32_t n_static) {
+  extern __shared__ bf16x8 lds[];  // kFragElems bf16 = 128 KiB
+  const uint32_t n = n_ptr ? *n_ptr : n_static;
+  const int wave = threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  const int h = lane / 32;
+  const int j = lane % 32;
+  const uint32_t rows_per_pass = 32u * kInferWaves;
+  for (uint32_t base = blockIdx.x * rows_per_pass; base < n; base += gridDim.x * rows_per_pass) {
+    const uint32_t row = base + wave * 32 + j;
+    const bool valid = row < n;
+    float xn[4];
+    for (int k = 0; k < 4; k++) {
+      const float v = valid ? x[size_t(row) * 4 + k] : 0.0f;
+      xn[k] = (v - m.in_shift[k]) * m.in_scale[k];
+    }
+    // layer 0 on the vector ALUs, straight into B-fragment order
+    bf16x8 B[16];
+#pragma unroll
+    for (int kb = 0; kb < 16; kb++) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float* w = m.p0 + ((h * 16 + kb) * 8 + q) * 5;
+        const float z = w[4] + w[0] * xn[0] + w[1] * xn[1] + w[2] * xn[2] + w[3] * xn[3];
+        B[kb][q] = (__bf16)Silu(z);
+      }
+    }
+    f32x16 acc[8];
+    // two 256 x 256 layers on the matrix cores
+#pragma unroll
+    for (int L = 0; L < 2; L++) {
+      const bf16x8* src = L == 0 ? m.frag1 : m.frag2;
+      __syncthreads();
+      for (int i = threadIdx.x; i < kFragElems / 8; i += blockDim.x) lds[i] = src[i];
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
+#pragma unroll
+      for (int kb = 0; kb < 16; kb++) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          const bf16x8 a = lds[(t * 16 + kb) * 64 + lane];
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kb], acc[t], 0, 0, 0);
+        }
+      }
+      const float* bias = L == 0 ? m.bias1 : m.bias2;
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const float a = Silu(acc[t][r] + bias[(t * 16 + r) * 2 + h]);
+          acc[t][r] = a;
+          if (L == 0) B[2 * t + r / 8][r % 8] = (__bf16)a;
+        }
+    }
+    // layer 3 (256 -> 2) from the FP32 activations, lane halves combined
+    float o0 = 0.0f, o1 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float* w = m.p3 + ((h * 8 + t) * 16 + r) * 2;
+        o0 += w[0] * acc[t][r];
+        o1 += w[1] * acc[t][r];
+      }
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 32);
+    if (valid && h == 0) {
+      y[size_t(row) * 2 + 0] = (o0 + m.b3[0]) * m.out_scale[0] + m.out_shift[0];
+      y[size_t(row) * 2 + 1] = (o1 + m.b3[1]) * m.out_scale[1] + m.out_shift[1];
+    }
+  }
+}
+
+// Packs the FP32 master parameters into the fused kernel's layouts.
+__global__ void KMlpPack(const float* P, int w0_off, int b0_off, int w1_off, int b1_off,
+                         int w2_off, int b2_off, int w3_off, __bf16* frag1, __bf16* frag2,
+                         float* p0, float* bias1, float* bias2, float* p3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kFragElems) {
+    // i = ((t * 16 + kb) * 64 + lane) * 8 + q
+    const int q = i % 8, lane = (i / 8) % 64, kb = (i / 512) % 16, t = i / 8192;
+    const int row = 32 * t + lane % 32;
+    const int col = FeatB(kb, lane / 32, q);
+    frag1[i] = (__bf16)P[w1_off + row * kHid + col];
+    frag2[i] = (__bf16)P[w2_off + row * kHid + col];
+  }
+  if (i < 2 * 16 * 8) {
+    const int q = i % 8, kb = (i / 8) % 16, h = i / 128;
+    const int f = FeatB(kb, h, q);
+    for (int k = 0; k < 4; k++) p0[i * 5 + k] = P[w0_off + f * 4 + k];
+    p0[i * 5 + 4] = P[b0_off + f];
+  }
+  if (i < 8 * 16 * 2) {
+    const int h = i % 2, r = (i / 2) % 16, t = i / 32;
+    bias1[i] = P[b1_off + FeatC(t, r, h)];
+    bias2[i] = P[b2_off + FeatC(t, r, h)];
+  }
+  if (i < 2 * 8 * 16) {
+    const int r = i % 16, t = (i / 16) % 8, h = i / 128;
+    const int f = FeatC(t, r, h);
+    p3[i * 2 + 0] = P[w3_off + 0 * kHid + f];
+    p3[i * 2 + 1] = P[w3_off + 1 * kHid + f];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The POD transport map (training targets).
+__device__ double BetaAtGrid(const DevWorld& W, const DTsl& t, int e_s, double T, double u) {
+  const double* Es = W.pool + t.Es_off;
+  int local;
+  const int p_i = PartitionOf(W.ipool + t.E_ends_off, t.n_bp, e_s, local);
+  const DTslPart P = W.parts[t.bp_begin + p_i];
+  const double* F = W.pool + P.F_off;
+  const int f_hi = UpperBoundArr(F, P.nF, u);
+  const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
+  const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
+  const double b_lo = f_hi != 0 ? PartEvaluate(W, P, f_hi - 1, local, T)
+                                : -Es[e_s] / (kBoltzmann * T);
+  const double b_hi = f_hi != P.nF ? PartEvaluate(W, P, f_hi, local, T) : t.beta_cutoff;
+  return b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
+}
+__device__ double AlphaFracAtGrid(const DevWorld& W, const DTsl& t, int b_i, int sgn, double b,
+                                  double E, double T, double u) {
+  const double* betas = W.pool + t.betas_off;
+  double b_s = sgn * betas[b_i];
+  if (E + b_s * kBoltzmann * T < 0) b_s = b;
+  const double sqrt_E = sqrt(E);
+  const double root = sqrt(E + b_s * kBoltzmann * T);
+  const double akT = t.awr * kBoltzmann * T;
+  const double a_min = (sqrt_E - root) * (sqrt_E - root) / akT;
+  const double a_max = (sqrt_E + root) * (sqrt_E + root) / akT;
+  int local;
+  const int p_i = PartitionOf(W.ipool + t.beta_ends_off, t.n_ap, b_i, local);
+  const DTslPart P = W.parts[t.ap_begin + p_i];
+  const double* F = W.pool + P.F_off;
+  const double F_min = FindCdf(W, P, local, T, a_min, t.alpha_cutoff);
+  const double F_max = FindCdf(W, P, local, T, a_max, t.alpha_cutoff);
+  const double Fq = F_min + u * (F_max - F_min);
+  const int hi = UpperBoundArr(F, P.nF, Fq);
+  const double F_lo = hi != 0 ? F[hi - 1] : 0;
+  const double F_hi = hi != P.nF ? F[hi] : 1;
+  const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
+  const double a_hi = hi != P.nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
+  const double a = a_lo + (Fq - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
+  const double fr = (a - a_min) / (a_max - a_min);
+  return fr < 0 ? 0 : (fr > 1 ? 1 : fr);
+}
+__device__ void PodMap(const DevWorld& W, const DTsl& t, double E, double T, double u1,
+                       double u2, double& y1, double& y2) {
+  const double* Es = W.pool + t.Es_off;
+  int e_hi = UpperBoundArr(Es, t.nEs, E);
+  if (e_hi >= t.nEs) e_hi = t.nEs - 1;
+  double b;
+  if (e_hi == 0) {
+    b = BetaAtGrid(W, t, 0, T, u1);
+  } else {
+    const double r = (E - Es[e_hi - 1]) / (Es[e_hi] - Es[e_hi - 1]);
+    b = (1 - r) * BetaAtGrid(W, t, e_hi - 1, T, u1) + r * BetaAtGrid(W, t, e_hi, T, u1);
+  }
+  const double b_min = -E / (kBoltzmann * T);
+  if (b < b_min) b = b_min;
+  if (b > t.beta_cutoff) b = t.beta_cutoff;
+  const double* betas = W.pool + t.betas_off;
+  const double ab = fabs(b);
+  const int sgn = (0 < b) - (b < 0);
+  int b_hi = UpperBoundArr(betas, t.nbetas, ab);
+  double fr;
+  if (b_hi == 0) {
+    fr = AlphaFracAtGrid(W, t, 0, sgn, b, E, T, u2);
+  } else if (b_hi >= t.nbetas) {
+    fr = AlphaFracAtGrid(W, t, t.nbetas - 1, sgn, b, E, T, u2);
+  } else {
+    const double r = (ab - betas[b_hi - 1]) / (betas[b_hi] - betas[b_hi - 1]);
+    fr = (1 - r) * AlphaFracAtGrid(W, t, b_hi - 1, sgn, b, E, T, u2) +
+         r * AlphaFracAtGrid(W, t, b_hi, sgn, b, E, T, u2);
+  }
+  y1 = b * (kBoltzmann * T) / (E + kBoltzmann * T);
+  y2 = fr;
+}
+// (y1, y2) -> (alpha, beta) in the trained parametrisation.
+__device__ __forceinline__ void OutputsToAlphaBeta(double y1, double y2, double E, double T,
+                                                   double awr, double beta_cutoff,
+                                                   double& alpha, double& beta) {
+  const double kT = kBoltzmann * T;
+  double b = y1 * (E + kT) / kT;
+  const double b_min = -E / kT;
+  if (b < b_min) b = b_min;
+  if (b > beta_cutoff) b = beta_cutoff;
+  const double Ep = E + b * kT;
+  const double sE = sqrt(E), sp = sqrt(Ep > 0 ? Ep : 0.0);
+  const double akT = awr * kT;
+  const double lo = (sE - sp) * (sE - sp) / akT, hi = (sE + sp) * (sE + sp) / akT;
+  const double f = y2 < 0 ? 0 : (y2 > 1 ? 1 : y2);
+  alpha = lo + f * (hi - lo);
+  beta = b;
+}
+
+__global__ void KPodMap(DevWorld W, int tsl, uint32_t n, const float* x, float* y) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DTsl t = W.tsl[tsl];
+  const double E = exp(double(x[size_t(i) * 4 + 0]));
+  const double T = x[size_t(i) * 4 + 1];
+  double y1, y2;
+  PodMap(W, t, E, T, x[size_t(i) * 4 + 2], x[size_t(i) * 4 + 3], y1, y2);
+  y[size_t(i) * 2 + 0] = float(y1);
+  y[size_t(i) * 2 + 1] = float(y2);
+}
+
+// ---------------------------------------------------------------------------
+// Tracker step with the surrogate.
+__device__ __forceinline__ bool InBox(const MlpState& m, double E, double T) {
+  return E <= m.e_max && T >= m.t_lo && T <= m.t_hi;
+}
+__global__ void __launch_bounds__(256) KTslGather(DevWorld W, RunState R, MlpState m,
+                                                  int iter) {
+  const uint32_t n = *Ring(R, iter, kT);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t p = R.QT[q];
+    const double E = R.B.E[p];
+    const double T = R.B.tslT[p];
+    uint32_t g = R.B.rng[p];
+    const double u1 = Canonical(g), u2 = Canonical(g);
+    R.B.rng[p] = g;
+    m.xin[size_t(q) * 4 + 0] = float(log(E));
+    m.xin[size_t(q) * 4 + 1] = float(T);
+    m.xin[size_t(q) * 4 + 2] = float(u1);
+    m.xin[size_t(q) * 4 + 3] = float(u2);
+  }
+}
+__global__ void __launch_bounds__(256) KTslApply(DevWorld W, RunState R, MlpState m, int iter,
+                                                 int qin) {
+  const uint32_t n = *Ring(R, iter, kT);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    bool alive = false, fallback = false;
+    uint32_t p = 0;
+    if (valid) {
+      p = R.QT[q];
+      V3 dir = LoadDir(R.B, p);
+      double E = R.B.E[p];
+      const double T = R.B.tslT[p];
+      uint32_t g = R.B.rng[p];
+      const DTsl t = W.tsl[R.B.tslI[p]];
+      double a = 0, b = 0;
+      uint32_t e = 0;
+      if (InBox(m, E, T)) {
+        OutputsToAlphaBeta(m.yout[size_t(q) * 2], m.yout[size_t(q) * 2 + 1], E, T, t.awr,
+                           t.beta_cutoff, a, b);
+        const double E_p = E + b * kBoltzmann * T;
+        const double mu = (E + E_p - a * t.awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
+        const double phi = 2 * kPi * Canonical(g);
+        dir = Rotate(dir, mu > 1 ? 1.0 : (mu < -1 ? -1.0 : mu), phi);
+        E = E_p > 0 ? E_p : 1e-300;
+      } else {
+        fallback = true;  // outside the trained box: the POD sampler
+        e = TslScatterPod(W, t, g, T, dir, E, a, b);
+      }
+      if (e) {
+        atomicOr(R.misc, e);
+      } else {
+        alive = true;
+        if (W.n_est_tsl) ScoreAlphaBeta(W, R, R.B.hist[p], a, b);
+        R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+        R.B.E[p] = E;
+        R.B.rng[p] = g;
+      }
+    }
+    uint32_t tot;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
+    if (alive) R.Q[qin ^ 1][iq] = p;
+    WaveCount(R.ev + kEvTsl, valid);
+    WaveCount(R.ev + kEvFallback, fallback);
+  }
+}
+
+static MlpDev DevView(const MlpState& m) {
+  MlpDev d;
+  d.frag1 = reinterpret_cast<const bf16x8*>(m.packed);
+  d.frag2 = reinterpret_cast<const bf16x8*>(m.packed) + kFragElems / 8;
+  const float* f = reinterpret_cast<const float*>(
+      reinterpret_cast<const char*>(m.packed) + 2 * kFragElems * sizeof(__bf16));
+  d.p0 = f;
+  d.bias1 = f + 2 * 16 * 8 * 5;
+  d.bias2 = d.bias1 + 8 * 16 * 2;
+  d.p3 = d.bias2 + 8 * 16 * 2;
+  d.b3[0] = m.b3[0];
+  d.b3[1] = m.b3[1];
+  for (int k = 0; k < 4; k++) {
+    d.in_shift[k] = m.in_shift[k];
+    d.in_scale[k] = m.in_scale[k];
+  }
+  for (int k = 0; k < 2; k++) {
+    d.out_shift[k] = m.out_shift[k];
+    d.out_scale[k] = m.out_scale[k];
+  }
+  return d;
+}
+
+static void LaunchFused(const MlpState& m, const float* x, float* y, const uint32_t* n_ptr,
+                        uint32_t n_static, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(KMlpFused, dim3(grid), dim3(64 * kInferWaves),
+                     kFragElems * sizeof(__bf16), s, DevView(m), x, y, n_ptr, n_static);
+}
+
+int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
+               uint32_t grid, hipStream_t s) {
+  if (!m.ready || !m.xin) return 1;
+  hipLaunchKernelGGL(KTslGather, dim3(grid), dim3(256), 0, s, W, R, m, iter);
+  // the fused kernel reads the queue length itself; 256 blocks = 1 per CU
+  LaunchFused(m, m.xin, m.yout, Ring(R, iter, kT), 0, 256, s);
+  hipLaunchKernelGGL(KTslApply, dim3(grid), dim3(256), 0, s, W, R, m, iter, qin);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 training kernels.
+// C[M x N] = op(A)[M x K] op(B)[K x N] (+ bias[N]); optional SiLU copy.
+// op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k]
+// op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n]
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+    KGemm(int M, int N, int K, const float* __restrict__ A, int lda,
+          const float* __restrict__ Bm, int ldb, float* __restrict__ C, int ldc,
+          const float* __restrict__ bias, float* __restrict__ H) {
+  __shared__ float As[16][64 + 4];
+  __shared__ float Bs[16][64 + 4];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+      int mm, kk;
+      if (TA) {
+        mm = e % 64;
+        kk = e / 64;
+      } else {
+        kk = e % 16;
+        mm = e / 16;
+      }
+      const int gm = m0 + mm, gk = k0 + kk;
+      float v = 0.0f;
+      if (gm < M && gk < K) v = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
+      As[kk][mm] = v;
+      int nn, kb;
+      if (TB) {
+        kb = e % 16;
+        nn = e / 16;
+      } else {
+        nn = e % 64;
+        kb = e / 64;
+      }
+      const int gn = n0 + nn, gk2 = k0 + kb;
+      float w = 0.0f;
+      if (gn < N && gk2 < K) w = TB ? Bm[size_t(gn) * ldb + gk2] : Bm[size_t(gk2) * ldb + gn];
+      Bs[kb][nn] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int i = 0; i < 4; i++) b[i] = Bs[kk][tx * 4 + i];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) acc[i][jj] += a[i] * b[jj];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int gm = m0 + ty * 4 + i;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int gn = n0 + tx * 4 + jj;
+      if (gn >= N) continue;
+      float v = acc[i][jj];
+      if (bias) v += bias[gn];
+      C[size_t(gm) * ldc + gn] = v;
+      if (H) H[size_t(gm) * ldc + gn] = v / (1.0f + expf(-v));
+    }
+  }
+}
+
+template <bool TA, bool TB>
+static void Gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B,
+                 int ldb, float* C, int ldc, const float* bias = nullptr, float* H = nullptr) {
+  dim3 grid((N + 63) / 64, (M + 63) / 64);
+  hipLaunchKernelGGL((KGemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc,
+                     bias, H);
+}
+
+// dZ = dH * silu'(Z)
+__global__ void KSiluBack(const float* Z, const float* dH, float* dZ, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const float z = Z[i];
+    const float sg = 1.0f / (1.0f + expf(-z));
+    dZ[i] = dH[i] * sg * (1.0f + z * (1.0f - sg));
+  }
+}
+// dZ = (Z - Y) / (rows * outs); loss partial sums
+__global__ void KLossGrad(const float* Z, const float* Y, float* dZ, size_t n, float inv,
+                          float* loss) {
+  float acc = 0.0f;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const float d = Z[i] - Y[i];
+    dZ[i] = d * inv;
+    acc += d * d;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (threadIdx.x % 64 == 0) atomicAdd(loss, acc * 0.5f * inv);
+}
+// db[n] += sum over rows of dZ[row][n]
+__global__ void KColSum(const float* dZ, int rows, int N, float* db) {
+  const int n = blockIdx.x * 64 + threadIdx.x % 64;
+  const int r0 = blockIdx.y * 1024 + threadIdx.x / 64;
+  float acc = 0.0f;
+  if (n < N)
+    for (int r = r0; r < min(rows, int(blockIdx.y + 1) * 1024); r += 4) acc += dZ[size_t(r) * N + n];
+  __shared__ float s[256];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64 && n < N)
+    atomicAdd(db + n, s[threadIdx.x] + s[threadIdx.x + 64] + s[threadIdx.x + 128] +
+                          s[threadIdx.x + 192]);
+}
+__global__ void KAdam(float* p, const float* g, float* m, float* v, size_t n, float lr,
+                      float b1, float b2, float eps, float c1, float c2) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1 - b1) * gi;
+    const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+__global__ void KZero(float* p, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = 0.0f;
+}
+// x_norm = (x - shift) * scale, y_norm = (y - shift) / scale, in place
+__global__ void KNormalize(float* x, float* y, size_t n, MlpDev d) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    for (int k = 0; k < 4; k++) x[i * 4 + k] = (x[i * 4 + k] - d.in_shift[k]) * d.in_scale[k];
+    if (y)
+      for (int k = 0; k < 2; k++) y[i * 2 + k] = (y[i * 2 + k] - d.out_shift[k]) / d.out_scale[k];
+  }
+}
+
 }  // namespace mmc
 
-extern "C" {
-int mmc_mlp_set(mmc_ctx*, int, const mmc_mlp_desc*, const float*, uint64_t) {
-  return MMC_ERR_UNSUPPORTED;
+// ---------------------------------------------------------------------------
+// Host side
+static bool FusedShape(const MlpState& m) {
+  return m.n_layers == 4 && m.widths[0] == 4 && m.widths[1] == kHid && m.widths[2] == kHid &&
+         m.widths[3] == kHid && m.widths[4] == 2;
 }
-int mmc_mlp_get(mmc_ctx*, float*, uint64_t) { return MMC_ERR_UNSUPPORTED; }
-int mmc_mlp_infer(mmc_ctx*, uint64_t, const float*, float*) { return MMC_ERR_UNSUPPORTED; }
-int mmc_tsl_sample_surrogate(mmc_ctx*, int, uint64_t, const double*, const double*,
-                             uint32_t*, double*, double*) {
-  return MMC_ERR_UNSUPPORTED;
+
+static int Repack(mmc_ctx* ctx) {
+  MlpState& m = MmcMlp(ctx);
+  if (!FusedShape(m)) return MMC_OK;
+  const int total = kFragElems;
+  char* base = reinterpret_cast<char*>(m.packed);
+  __bf16* f1 = reinterpret_cast<__bf16*>(base);
+  __bf16* f2 = f1 + kFragElems;
+  float* f = reinterpret_cast<float*>(base + 2 * kFragElems * sizeof(__bf16));
+  float* p0 = f;
+  float* b1 = p0 + 2 * 16 * 8 * 5;
+  float* b2 = b1 + 8 * 16 * 2;
+  float* p3 = b2 + 8 * 16 * 2;
+  hipLaunchKernelGGL(KMlpPack, dim3((total + 255) / 256), dim3(256), 0, MmcStream(ctx),
+                     m.params, m.w_off[0], m.b_off[0], m.w_off[1], m.b_off[1], m.w_off[2],
+                     m.b_off[2], m.w_off[3], f1, f2, p0, b1, b2, p3);
+  // layer-3 bias lives in the kernel arguments
+  float b3[2];
+  if (hipMemcpyAsync(b3, m.params + m.b_off[3], sizeof(b3), hipMemcpyDeviceToHost,
+                     MmcStream(ctx)) != hipSuccess ||
+      hipStreamSynchronize(MmcStream(ctx)) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "mlp repack failed");
+  m.b3[0] = b3[0];
+  m.b3[1] = b3[1];
+  return MMC_OK;
 }
-int mmc_mlp_train(mmc_ctx*, uint64_t, const float*, const float*, const mmc_train_opts*,
-                  float*) {
-  return MMC_ERR_UNSUPPORTED;
+
+extern "C" int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* d, const float* params,
+                           uint64_t n_params) {
+  if (!ctx || !d || d->n_layers < 1 || d->n_layers > kMlpMaxLayers) return MMC_ERR_ARG;
+  if (hipSetDevice(MmcDevice(ctx)) != hipSuccess) return MMC_ERR_HIP;
+  MlpState& m = MmcMlp(ctx);
+  MlpFree(m);
+  m.tsl = tsl;
+  m.n_layers = d->n_layers;
+  long off = 0;
+  for (int l = 0; l <= d->n_layers; l++) {
+    if (d->widths[l] < 1) return MmcFail(ctx, MMC_ERR_ARG, "bad mlp width");
+    m.widths[l] = d->widths[l];
+  }
+  for (int l = 0; l < d->n_layers; l++) {
+    m.w_off[l] = int(off);
+    off += long(m.widths[l + 1]) * m.widths[l];
+    m.b_off[l] = int(off);
+    off += m.widths[l + 1];
+  }
+  m.n_params = off;
+  if (params && n_params != uint64_t(off)) return MmcFail(ctx, MMC_ERR_ARG, "params size");
+  std::memcpy(m.in_shift, d->in_shift, sizeof(m.in_shift));
+  std::memcpy(m.in_scale, d->in_scale, sizeof(m.in_scale));
+  std::memcpy(m.out_shift, d->out_shift, sizeof(m.out_shift));
+  std::memcpy(m.out_scale, d->out_scale, sizeof(m.out_scale));
+  m.e_max = d->e_max;
+  m.t_lo = d->t_lo;
+  m.t_hi = d->t_hi;
+  const size_t pbytes = sizeof(float) * off;
+  if (hipMalloc(&m.params, pbytes) != hipSuccess || hipMalloc(&m.grads, pbytes) != hipSuccess ||
+      hipMalloc(&m.adam_m, pbytes) != hipSuccess || hipMalloc(&m.adam_v, pbytes) != hipSuccess ||
+      hipMalloc(&m.loss, sizeof(float)) != hipSuccess ||
+      hipMalloc(&m.packed, 2 * kFragElems * sizeof(__bf16) + 4096 * sizeof(float)) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "mlp allocation failed");
+  hipMemsetAsync(m.adam_m, 0, pbytes, MmcStream(ctx));
+  hipMemsetAsync(m.adam_v, 0, pbytes, MmcStream(ctx));
+  m.adam_t = 0;
+  if (params) {
+    if (hipMemcpy(m.params, params, pbytes, hipMemcpyHostToDevice) != hipSuccess)
+      return MmcFail(ctx, MMC_ERR_HIP, "mlp upload failed");
+  } else {
+    // He-style uniform initialisation from a fixed LCG
+    std::vector<float> h(off);
+    uint64_t st = 0x9E3779B97F4A7C15ULL;
+    auto rnd = [&]() {
+      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+      return double(st >> 11) / double(1ULL << 53);
+    };
+    for (int l = 0; l < m.n_layers; l++) {
+      const double lim = std::sqrt(6.0 / m.widths[l]);
+      for (long i = 0; i < long(m.widths[l + 1]) * m.widths[l]; i++)
+        h[m.w_off[l] + i] = float((2 * rnd() - 1) * lim);
+      for (int i = 0; i < m.widths[l + 1]; i++) h[m.b_off[l] + i] = 0.0f;
+    }
+    if (hipMemcpy(m.params, h.data(), pbytes, hipMemcpyHostToDevice) != hipSuccess)
+      return MmcFail(ctx, MMC_ERR_HIP, "mlp upload failed");
+  }
+  m.ready = FusedShape(m);
+  const int rc = Repack(ctx);
+  if (rc) return rc;
+  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
 }
-int mmc_mlp_grad(mmc_ctx*, uint64_t, const float*, const float*, float*, float*) {
-  return MMC_ERR_UNSUPPORTED;
+
+extern "C" int mmc_mlp_get(mmc_ctx* ctx, float* params, uint64_t n_params) {
+  if (!ctx || !params) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.params || n_params != uint64_t(m.n_params)) return MMC_ERR_ARG;
+  if (hipStreamSynchronize(MmcStream(ctx)) != hipSuccess ||
+      hipMemcpy(params, m.params, sizeof(float) * m.n_params, hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    return MMC_ERR_HIP;
+  return MMC_OK;
 }
+
+extern "C" int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y) {
+  if (!ctx || !x || !y) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.ready) return MmcFail(ctx, MMC_ERR_UNSUPPORTED, "no 4-256-256-256-2 network set");
+  if (n == 0) return MMC_OK;
+  if (n > 0xffffffffull) return MmcFail(ctx, MMC_ERR_ARG, "batch too large");
+  const uint32_t rows_per_block = 32u * kInferWaves;
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + rows_per_block - 1) / rows_per_block, 1024));
+  LaunchFused(m, x, y, nullptr, uint32_t(n), grid, MmcStream(ctx));
+  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+}
+
+extern "C" int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y) {
+  if (!ctx || !x || !y || tsl < 0) return MMC_ERR_ARG;
+  if (n == 0) return MMC_OK;
+  if (n > 0xffffffffull) return MMC_ERR_ARG;
+  hipLaunchKernelGGL(KPodMap, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, MmcStream(ctx),
+                     MmcWorld(ctx), tsl, uint32_t(n), x, y);
+  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+}
+
+// Forward + backward of one minibatch (normalised x, y) into m.grads.
+static int ForwardBackward(mmc_ctx* ctx, uint64_t rows, const float* x, const float* y) {
+  MlpState& m = MmcMlp(ctx);
+  hipStream_t s = MmcStream(ctx);
+  const int L = m.n_layers;
+  int wmax = 0;
+  for (int l = 0; l <= L; l++) wmax = std::max(wmax, m.widths[l]);
+  const size_t need = size_t(rows) * wmax * (3 * L + 2);
+  if (m.act_cap < need) {
+    if (m.act) (void)hipFree(m.act);
+    m.act = nullptr;
+    if (hipMalloc(&m.act, need * sizeof(float)) != hipSuccess)
+      return MmcFail(ctx, MMC_ERR_HIP, "activation allocation failed");
+    m.act_cap = need;
+  }
+  // per layer l: Z_l and H_l (rows x widths[l+1]); dZ scratch, dH scratch
+  std::vector<float*> Z(L), H(L);
+  float* p = m.act;
+  for (int l = 0; l < L; l++) {
+    Z[l] = p;
+    p += size_t(rows) * wmax;
+    H[l] = p;
+    p += size_t(rows) * wmax;
+  }
+  float* dZ = p;
+  p += size_t(rows) * wmax;
+  float* dH = p;
+  const int R = int(rows);
+  for (int l = 0; l < L; l++) {
+    const float* in = l == 0 ? x : H[l - 1];
+    Gemm<false, true>(s, R, m.widths[l + 1], m.widths[l], in, m.widths[l], m.params + m.w_off[l],
+                      m.widths[l], Z[l], m.widths[l + 1], m.params + m.b_off[l],
+                      l + 1 < L ? H[l] : nullptr);
+  }
+  hipLaunchKernelGGL(KZero, dim3(256), dim3(256), 0, s, m.grads, size_t(m.n_params));
+  hipLaunchKernelGGL(KZero, dim3(1), dim3(64), 0, s, m.loss, size_t(1));
+  const int no = m.widths[L];
+  hipLaunchKernelGGL(KLossGrad, dim3(1024), dim3(256), 0, s, Z[L - 1], y, dZ, size_t(R) * no,
+                     1.0f / float(R * no), m.loss);
+  for (int l = L - 1; l >= 0; l--) {
+    const int nout = m.widths[l + 1], nin = m.widths[l];
+    const float* in = l == 0 ? x : H[l - 1];
+    // dW_l = dZ^T in
+    Gemm<true, false>(s, nout, nin, R, dZ, nout, in, nin, m.grads + m.w_off[l], nin);
+    hipLaunchKernelGGL(KColSum, dim3((nout + 63) / 64, (R + 1023) / 1024), dim3(256), 0, s, dZ,
+                       R, nout, m.grads + m.b_off[l]);
+    if (l > 0) {
+      // dH_{l-1} = dZ W_l ; dZ_{l-1} = dH * silu'(Z_{l-1})
+      Gemm<false, false>(s, R, nin, nout, dZ, nout, m.params + m.w_off[l], nin, dH, nin);
+      hipLaunchKernelGGL(KSiluBack, dim3(1024), dim3(256), 0, s, Z[l - 1], dH, dZ,
+                         size_t(R) * nin);
+    }
+  }
+  return MMC_OK;
+}
+
+extern "C" int mmc_mlp_grad(mmc_ctx* ctx, uint64_t n, const float* x_host, const float* y_host,
+                            float* loss, float* grads) {
+  if (!ctx || !x_host || !y_host || n == 0) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.params) return MMC_ERR_ARG;
+  const int ni = m.widths[0], no = m.widths[m.n_layers];
+  float *dx = nullptr, *dy = nullptr;
+  if (hipMalloc(&dx, sizeof(float) * n * ni) != hipSuccess ||
+      hipMalloc(&dy, sizeof(float) * n * no) != hipSuccess)
+    return MMC_ERR_HIP;
+  hipMemcpy(dx, x_host, sizeof(float) * n * ni, hipMemcpyHostToDevice);
+  hipMemcpy(dy, y_host, sizeof(float) * n * no, hipMemcpyHostToDevice);
+  int rc = ForwardBackward(ctx, n, dx, dy);
+  if (!rc) {
+    hipStreamSynchronize(MmcStream(ctx));
+    if (loss) hipMemcpy(loss, m.loss, sizeof(float), hipMemcpyDeviceToHost);
+    if (grads) hipMemcpy(grads, m.grads, sizeof(float) * m.n_params, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return rc;
+}
+
+extern "C" int mmc_mlp_train(mmc_ctx* ctx, uint64_t n, const float* x, const float* y,
+                             const mmc_train_opts* o, float* last_loss) {
+  if (!ctx || !x || !y || !o || o->batch == 0 || o->batch > n) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.params) return MMC_ERR_ARG;
+  hipStream_t s = MmcStream(ctx);
+  uint64_t off = o->offset % n;
+  const int ni = m.widths[0], no = m.widths[m.n_layers];
+  for (uint64_t step = 0; step < o->steps; step++) {
+    if (off + o->batch > n) off = 0;
+    int rc = ForwardBackward(ctx, o->batch, x + off * ni, y + off * no);
+    if (rc) return rc;
+    m.adam_t++;
+    const float c1 = 1.0f - std::pow(o->beta1, float(m.adam_t));
+    const float c2 = 1.0f - std::pow(o->beta2, float(m.adam_t));
+    hipLaunchKernelGGL(KAdam, dim3(256), dim3(256), 0, s, m.params, m.grads, m.adam_m, m.adam_v,
+                       size_t(m.n_params), o->lr, o->beta1, o->beta2, o->eps, c1, c2);
+    off += o->batch;
+  }
+  const int rc = Repack(ctx);
+  if (rc) return rc;
+  if (last_loss) hipMemcpyAsync(last_loss, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
+  return hipStreamSynchronize(s) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+}
+
+// Device batch: normalise raw inputs/targets in place (training data prep).
+extern "C" int mmc_mlp_normalize(mmc_ctx* ctx, uint64_t n, float* x, float* y) {
+  if (!ctx || !x) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.params) return MMC_ERR_ARG;
+  hipLaunchKernelGGL(KNormalize, dim3(1024), dim3(256), 0, MmcStream(ctx), x, y, size_t(n),
+                     DevView(m));
+  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+}
+
+__global__ void KSurrogateBatch(DevWorld W, MlpState m, int tsl, uint32_t n, const double* E,
+                                const double* T, uint32_t* rng, float* xin) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t g = rng[i];
+  const double u1 = Canonical(g), u2 = Canonical(g);
+  rng[i] = g;
+  xin[size_t(i) * 4 + 0] = float(log(E[i]));
+  xin[size_t(i) * 4 + 1] = float(T[i]);
+  xin[size_t(i) * 4 + 2] = float(u1);
+  xin[size_t(i) * 4 + 3] = float(u2);
+}
+__global__ void KSurrogateFinish(DevWorld W, MlpState m, int tsl, uint32_t n, const double* E,
+                                 const double* T, uint32_t* rng, const float* yout,
+                                 double* alpha, double* beta, uint32_t* err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DTsl t = W.tsl[tsl];
+  double a, b;
+  if (InBox(m, E[i], T[i])) {
+    OutputsToAlphaBeta(yout[size_t(i) * 2], yout[size_t(i) * 2 + 1], E[i], T[i], t.awr,
+                       t.beta_cutoff, a, b);
+  } else {
+    uint32_t g = rng[i];
+    uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
+    if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);
+    if (e) {
+      atomicOr(err, e);
+      a = b = __builtin_nan("");
+    }
+    rng[i] = g;
+  }
+  alpha[i] = a;
+  beta[i] = b;
+}
+
+extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
+                                        const double* T, uint32_t* rng, double* alpha,
+                                        double* beta) {
+  if (!ctx || !E || !T || !rng || !alpha || !beta || tsl < 0) return MMC_ERR_ARG;
+  MlpState& m = MmcMlp(ctx);
+  if (!m.ready) return MmcFail(ctx, MMC_ERR_UNSUPPORTED, "no surrogate network set");
+  if (n == 0) return MMC_OK;
+  if (n > 0xffffffffull) return MMC_ERR_ARG;
+  hipStream_t s = MmcStream(ctx);
+  float *xin = nullptr, *yout = nullptr;
+  uint32_t* err = nullptr;
+  if (hipMalloc(&xin, sizeof(float) * 4 * n) != hipSuccess ||
+      hipMalloc(&yout, sizeof(float) * 2 * n) != hipSuccess ||
+      hipMalloc(&err, sizeof(uint32_t)) != hipSuccess)
+    return MMC_ERR_HIP;
+  hipMemsetAsync(err, 0, sizeof(uint32_t), s);
+  const uint32_t nb = uint32_t((n + 255) / 256);
+  hipLaunchKernelGGL(KSurrogateBatch, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
+                     uint32_t(n), E, T, rng, xin);
+  const uint32_t rows_per_block = 32u * kInferWaves;
+  LaunchFused(m, xin, yout, nullptr, uint32_t(n),
+              uint32_t(std::min<uint64_t>((n + rows_per_block - 1) / rows_per_block, 1024)), s);
+  hipLaunchKernelGGL(KSurrogateFinish, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
+                     uint32_t(n), E, T, rng, yout, alpha, beta, err);
+  uint32_t herr = 0;
+  hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  const bool ok = hipStreamSynchronize(s) == hipSuccess;
+  (void)hipFree(xin);
+  (void)hipFree(yout);
+  (void)hipFree(err);
+  if (!ok) return MMC_ERR_HIP;
+  if (herr) return MmcFail(ctx, MMC_ERR_RESAMPLE, "POD fallback failed");
+  return MMC_OK;
+}
+
+// Staging buffers for the tracker's surrogate step (sized to the bank).
+int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap) {
+  MlpState& m = MmcMlp(ctx);
+  if (m.stage_cap >= cap) return MMC_OK;
+  if (m.xin) (void)hipFree(m.xin);
+  if (m.yout) (void)hipFree(m.yout);
+  m.xin = nullptr;
+  m.yout = nullptr;
+  if (hipMalloc(&m.xin, sizeof(float) * 4 * size_t(cap)) != hipSuccess ||
+      hipMalloc(&m.yout, sizeof(float) * 2 * size_t(cap)) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "surrogate staging allocation failed");
+  m.stage_cap = cap;
+  return MMC_OK;
 }

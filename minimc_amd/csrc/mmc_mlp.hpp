@@ -19,7 +19,9 @@ struct MlpState {
   float in_shift[4] = {}, in_scale[4] = {}, out_shift[2] = {}, out_scale[2] = {};
   double e_max = 0, t_lo = 0, t_hi = 0;
   float* params = nullptr;        // FP32 master parameters
-  unsigned short* wbf16 = nullptr;  // BF16 weights in the inference layout
+  void* wbf16 = nullptr;          // unused
+  void* packed = nullptr;         // inference layouts (BF16 fragments + FP32)
+  float b3[2] = {};
   float *adam_m = nullptr, *adam_v = nullptr, *grads = nullptr;
   float* act = nullptr;           // training activations
   size_t act_cap = 0;
@@ -32,6 +34,11 @@ struct MlpState {
 };
 
 void MlpFree(MlpState& m);
+}  // namespace mmc
+struct mmc_ctx;
+// Staging buffers of the surrogate step for `cap` particle slots.
+int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap);
+namespace mmc {
 // One thermal-scattering step of the tracker with the surrogate: gathers
 // (log E, T, u1, u2) of the TSL queue, runs the network, applies the
 // scatter, and sends out-of-box inputs to the POD sampler.

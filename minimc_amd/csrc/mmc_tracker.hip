@@ -773,6 +773,12 @@ extern "C" int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* la
 // from device memory and grid-stride over them.
 static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   const int K = ctx->opt.graph_iters > 0 ? ctx->opt.graph_iters : 8;
+  if (ctx->has_tsl && ctx->opt.tsl_mode == MMC_TSL_SURROGATE) {
+    if (!ctx->mlp.ready)
+      return Fail(ctx, MMC_ERR_UNSUPPORTED, "surrogate mode without a network (mmc_mlp_set)");
+    const int rc = MlpEnsureStaging(ctx, ctx->cap);
+    if (rc) return rc;
+  }
   uint32_t bound = n0;
   int iter = 0, qin = 0;
   std::vector<uint32_t> ring(kRing * kRingStride);

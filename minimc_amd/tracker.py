@@ -57,6 +57,8 @@ _SYMBOLS = {
     "mmc_tsl_sample_surrogate": ([C.c_void_p, C.c_int, C.c_uint64, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
                                  C.c_int),
+    "mmc_tsl_pod_map": ([C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p], C.c_int),
+    "mmc_mlp_normalize": ([C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p], C.c_int),
     "mmc_mlp_train": ([C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                        C.POINTER(abi.TrainOpts), C.POINTER(C.c_float)], C.c_int),
     "mmc_mlp_grad": ([C.c_void_p, C.c_uint64, abi.c_float_p, abi.c_float_p,

@@ -1,0 +1,132 @@
+"""Thermal-scattering surrogate on the GPU: training gradients and fused
+BF16-MFMA inference against the FP64 oracle, the POD transport map against
+the POD tables, and the trained sampler against the POD sampler."""
+import numpy as np
+import pytest
+
+from minimc_amd import abi, configs, surrogate
+from oracle import mlp_ref
+from oracle.oracle import Oracle
+from tests.gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+K = 8.617333262145e-11
+
+
+@pytest.fixture(scope="module")
+def env():
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = configs.tsl_box(configs.tsl_tables(), T=600.0)
+    t = Tracker(pr, max_in_flight=1 << 16)
+    return pr, t, surrogate.Surrogate(t)
+
+
+def _random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = []
+    w = surrogate.WIDTHS
+    for i in range(len(w) - 1):
+        p.append(rng.uniform(-1, 1, w[i + 1] * w[i]) * scale * np.sqrt(3.0 / w[i]))
+        p.append(rng.uniform(-0.1, 0.1, w[i + 1]))
+    return np.concatenate(p).astype(np.float32)
+
+
+def test_training_gradients_match_oracle(env):
+    pr, t, s = env
+    p = _random_params(0)
+    s.set(p)
+    rng = np.random.default_rng(1)
+    n = 300  # not a multiple of any tile
+    x = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
+    y = rng.normal(size=(n, 2)).astype(np.float32)
+    loss, g = s.grad(x, y)
+    rl, rg = mlp_ref.loss_and_grad(p.astype(np.float64), surrogate.WIDTHS,
+                                   x.astype(np.float64), y.astype(np.float64))
+    assert loss == pytest.approx(rl, rel=1e-4)
+    scale = np.abs(rg).max()
+    assert np.max(np.abs(g - rg)) <= 2e-3 * scale
+    # per layer relative error
+    off = 0
+    w = surrogate.WIDTHS
+    for i in range(len(w) - 1):
+        m = w[i + 1] * w[i] + w[i + 1]
+        a, b = g[off:off + m], rg[off:off + m]
+        assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b) + 1e-9
+        off += m
+
+
+def test_fused_inference_matches_oracle(env):
+    pr, t, s = env
+    p = _random_params(2)
+    desc = surrogate.make_desc(out_shift=(0.1, 0.5), out_scale=(2.0, 0.3))
+    s.set(p, desc)
+    n = 4096 + 37
+    x = surrogate.sample_inputs(n, 4)
+    xd = t.device_array(x)
+    y = s.infer(xd, n).numpy()
+    xn = (x.astype(np.float64) - np.array(desc.in_shift)) * np.array(desc.in_scale)
+    ref = mlp_ref.forward(p.astype(np.float64), surrogate.WIDTHS, xn)
+    ref = ref * np.array(desc.out_scale) + np.array(desc.out_shift)
+    err = np.abs(y - ref)
+    spread = ref.std(0)
+    # BF16 operands, FP32 accumulation
+    assert np.all(err.max(0) <= 0.05 * spread), (err.max(0), spread)
+    assert np.all(err.mean(0) <= 0.01 * spread), (err.mean(0), spread)
+
+
+def test_pod_map_reproduces_tables(env):
+    pr, t, s = env
+    tab = configs.tsl_tables()
+    o = Oracle(pr)
+    part = tab["beta_partitions"][1]
+    Ts = tab["total_T_axis"]
+    rows = []
+    for (i, j, k) in ((3, 50, 2), (10, 120, 10), (20, 180, 20)):
+        E = part["x"][i]
+        rows.append((np.log(E), Ts[k], part["F"][j], 0.5, i, j, k))
+    x = np.array([r[:4] for r in rows], dtype=np.float32)
+    y = s.pod_map(t.device_array(x), len(rows)).numpy()
+    for q, (_, T, F, _, i, j, k) in enumerate(rows):
+        E = float(np.exp(np.float64(x[q, 0])))
+        beta = o.tsl_evaluate(0, 1, j, i, T)
+        want = beta * K * T / (E + K * T)
+        assert y[q, 0] == pytest.approx(want, rel=2e-4, abs=1e-6)
+    # monotone in u1, fraction in [0, 1]
+    n = 2000
+    x = np.zeros((n, 4), dtype=np.float32)
+    x[:, 0] = np.log(0.7e-6)
+    x[:, 1] = 750.0
+    x[:, 2] = np.linspace(0.0005, 0.9995, n)
+    x[:, 3] = 0.3
+    y = s.pod_map(t.device_array(x), n).numpy()
+    assert np.all(np.diff(y[:, 0]) >= -1e-6)
+    assert np.all((y[:, 1] >= 0) & (y[:, 1] <= 1))
+
+
+def _w1_rel(a, b):
+    q = np.linspace(0, 1, 2001)[1:-1]
+    return np.mean(np.abs(np.quantile(a, q) - np.quantile(b, q))) / np.std(b)
+
+
+def test_trained_surrogate_matches_pod_sampler(env):
+    pr, t, s = env
+    losses = s.train(n_samples=1 << 21, batch=8192, epochs=4, lr=2e-3, seed=5)
+    assert losses[-1] < 0.2 * losses[0]
+    n = 200000
+    for E_eV, T in ((0.02, 350.0), (0.6, 800.0), (3.0, 1150.0)):
+        E = t.device_array(np.full(n, E_eV * 1e-6))
+        Tt = t.device_array(np.full(n, T))
+        rng0 = np.random.default_rng(9).integers(1, 2**31 - 1, n).astype(np.uint32)
+        r1, r2 = t.device_array(rng0), t.device_array(rng0)
+        a1 = t.device_array(shape=n, dtype=np.float64)
+        b1 = t.device_array(shape=n, dtype=np.float64)
+        a2 = t.device_array(shape=n, dtype=np.float64)
+        b2 = t.device_array(shape=n, dtype=np.float64)
+        t.tsl_sample_pod(E, Tt, r1, a1, b1)
+        t.tsl_sample_surrogate(E, Tt, r2, a2, b2)
+        ap, bp, am, bm = a1.numpy(), b1.numpy(), a2.numpy(), b2.numpy()
+        assert np.all(np.isfinite(am)) and np.all(np.isfinite(bm))
+        # a small training run: a loose bound (the full run is judged in bench)
+        assert _w1_rel(bm, bp) < 0.15, (E_eV, T, _w1_rel(bm, bp))
+        assert _w1_rel(am, ap) < 0.15, (E_eV, T, _w1_rel(am, ap))

@@ -1,0 +1,43 @@
+"""The FP64 MLP oracle (oracle/mlp_ref.py) against central finite
+differences, and the surrogate's host-side helpers."""
+import numpy as np
+
+from minimc_amd import surrogate
+from oracle import mlp_ref
+
+
+def test_gradient_matches_finite_differences():
+    widths = (4, 7, 5, 2)
+    rng = np.random.default_rng(0)
+    n_p = surrogate.n_params(widths)
+    p = rng.normal(size=n_p) * 0.5
+    x = rng.normal(size=(13, 4))
+    y = rng.normal(size=(13, 2))
+    loss, g = mlp_ref.loss_and_grad(p, widths, x, y)
+    eps = 1e-6
+    for i in rng.choice(n_p, 40, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (mlp_ref.loss_and_grad(pp, widths, x, y)[0] -
+              mlp_ref.loss_and_grad(pm, widths, x, y)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd)
+    # loss definition: sum of squares / (2 n n_out)
+    out = mlp_ref.forward(p, widths, x)
+    assert loss == np.sum((out - y) ** 2) / (2 * 13 * 2)
+
+
+def test_parameter_count_and_layout():
+    assert surrogate.n_params() == 133378   # 4-256-256-256-2
+    p = np.arange(surrogate.n_params(), dtype=np.float64)
+    layers = mlp_ref.unpack(p, surrogate.WIDTHS)
+    assert layers[0][0].shape == (256, 4) and layers[0][0][0, 1] == 1.0
+    assert layers[0][1][0] == 1024.0
+    assert layers[-1][0].shape == (2, 256)
+
+
+def test_input_sampling_box():
+    x = surrogate.sample_inputs(100000, 3)
+    assert x[:, 0].min() >= np.log(1e-11) - 1e-5 and x[:, 0].max() <= np.log(4e-6) + 1e-5
+    assert x[:, 1].min() >= 300 and x[:, 1].max() <= 1200
+    assert 0 <= x[:, 2].min() and x[:, 3].max() < 1
