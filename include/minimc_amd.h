@@ -410,8 +410,10 @@ enum mmc_kernel_kind {
   MMC_K_ADVANCE = 0, /* XS lookup + collision distance + CSG distance */
   MMC_K_CROSS = 1,   /* surface crossing + cell search + current tally */
   MMC_K_COLLIDE = 2, /* nuclide + reaction sampling, collision physics */
-  MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling      */
-  MMC_K_N = 4
+  MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling: the
+                        POD sampler, or the surrogate's gather + apply */
+  MMC_K_MLP = 4,     /* the surrogate's fused BF16-MFMA network        */
+  MMC_K_N = 5
 };
 int mmc_set_profiling(mmc_ctx* ctx, int enable);
 /* total_ms[k] and launches[k] for k < MMC_K_N (accumulated since enable). */

@@ -219,3 +219,100 @@ def c3_broomstick(T=450.0, radius=0.01, histories=10_000_000, tables=None):
                               energy=dict(kind=abi.MMC_BINS_LOGSPACE, min=-11,
                                           max=float(np.log10(4e-6)), bins=100)))
     return pr
+
+
+# ---------------------------------------------------------------------------
+# Multi-zone k-eigenvalue: a 60 cm cube split into 27 cubes of 20 cm
+# (plane-x/y/z at -30, -10, 10, 30 cm) with vacuum outside.  Corners (8)
+# hold a structural scatterer (the awr-10 nuclide of the CE slab), the six
+# face-centre cubes hold fuel (a fictitious fissile nuclide: the same kind of
+# synthetic cross sections plus 1/v fission, nu = 2.4, Watt spectrum
+# a = 0.988 MeV, b = 2.249 /MeV), the twelve edge cubes and the centre hold
+# moderator (bound hydrogen with the continuous-temperature thermal
+# scattering law).  Each cube has its own temperature, uniform in
+# [300, 1200] K from a fixed seed.  Initial source: uniform in the cube,
+# isotropic, 2 MeV.
+C5_EDGES = (-30.0, -10.0, 10.0, 30.0)
+C5_WATT = (0.988, 2.249)
+
+
+def fissile_tables(seed=31, awr=235.0, sigma_f0=60.0, nu=2.4, temperature=ROOM_TEMPERATURE):
+    from . import synth
+    base = synth.ce_nuclide(n=60000, n_res=200, seed=seed, sigma_c0=8.0, sigma_pot=10.0)
+    E = base["E"]
+    fis = sigma_f0 * np.sqrt(0.0253e-6 / E) + 1.0  # 1/v plus a 1 b fast floor
+    return dict(awr=awr, reactions=[
+        dict(kind=abi.MMC_RX_CAPTURE, xs=(E, base["capture"]), temperature=temperature),
+        dict(kind=abi.MMC_RX_SCATTER, xs=(E, base["elastic"]), temperature=temperature),
+        dict(kind=abi.MMC_RX_FISSION, xs=(E, fis), temperature=temperature,
+             nubar=(np.array([E[0], E[-1]]), np.array([nu, nu])),
+             chi_kind=abi.MMC_CHI_WATT, watt_a=C5_WATT[0], watt_b=C5_WATT[1])],
+        total=(E, base["capture"] + base["elastic"] + fis), total_temperature=temperature)
+
+
+def c5_moderator_nuclide(temperature=ROOM_TEMPERATURE):
+    """Bound hydrogen on the shared 60,000-point grid (1/v capture 0.33 b at
+    0.0253 eV, 20 b elastic above the thermal-scattering cutoff)."""
+    from . import synth
+    E = synth.log_grid(60000)
+    cap = 0.33 * np.sqrt(0.0253e-6 / E)
+    el = np.full_like(E, 20.0)
+    return dict(awr=H_AWR, total=(E, cap + el), total_temperature=temperature, reactions=[
+        dict(kind=abi.MMC_RX_CAPTURE, xs=(E, cap), temperature=temperature),
+        dict(kind=abi.MMC_RX_SCATTER, xs=(E, el), temperature=temperature, tsl=0)])
+
+
+def c5_multizone(histories=1_000_000, fuel_aden=0.02, mod_aden=H_ADEN, refl_aden=0.08,
+                 seed=20261202, tables=None):
+    from . import synth
+    t = tables if tables is not None else tsl_tables()
+    pr = Problem(abi.MMC_CONTINUOUS)
+    pr.add_field()  # global (room temperature); every cube overrides it
+    planes = {}
+    for ax, kind in enumerate((abi.MMC_PLANEX, abi.MMC_PLANEY, abi.MMC_PLANEZ)):
+        for e in C5_EDGES:
+            planes[(ax, e)] = pr.add_surface("p%d_%g" % (ax, e), kind, a=e)
+    pr.tsl.append(t)
+    pr.nuclides.append(fissile_tables())                      # 0 fuel
+    pr.nuclides.append(c5_moderator_nuclide())                # 1 bound H
+    pr.nuclides.append(ce_tables(synth.ce_nuclide()))         # 2 structure (awr 10)
+    fuel = pr.add_material("fuel", fuel_aden, [(0, 1.0)])
+    mod = pr.add_material("moderator", mod_aden, [(1, 1.0)])
+    refl = pr.add_material("structure", refl_aden, [(2, 1.0)])
+    rng = np.random.default_rng(seed)
+    temps = rng.uniform(300.0, 1200.0, 27)
+    e = C5_EDGES
+    n = 0
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                idx = (i, j, k)
+                n_mid = sum(1 for v in idx if v == 1)
+                mat = refl if n_mid == 0 else (fuel if n_mid == 2 else mod)
+                faces = []
+                for ax, c in enumerate(idx):
+                    faces.append((planes[(ax, e[c])], False))      # above lower plane
+                    faces.append((planes[(ax, e[c + 1])], True))   # below upper plane
+                f = pr.add_field(c=float(temps[n]))
+                pr.add_cell("cube_%d%d%d" % idx, mat, faces, f)
+                n += 1
+    lo, hi = e[0], e[-1]
+    px = [planes[(0, lo)], planes[(0, hi)]]
+    py = [planes[(1, lo)], planes[(1, hi)]]
+    pz = [planes[(2, lo)], planes[(2, hi)]]
+    pr.add_cell("", -1, [(px[0], True)])
+    pr.add_cell("", -1, [(px[1], False)])
+    inx = [(px[0], False), (px[1], True)]
+    pr.add_cell("", -1, inx + [(py[0], True)])
+    pr.add_cell("", -1, inx + [(py[1], False)])
+    iny = [(py[0], False), (py[1], True)]
+    pr.add_cell("", -1, inx + iny + [(pz[0], True)])
+    pr.add_cell("", -1, inx + iny + [(pz[1], False)])
+    pr.problem = "keigenvalue"
+    pr.inactive, pr.active = 20, 100
+    pr.histories = histories
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_ISOTROPIC, E=2.0,
+                     box=((lo, lo, lo), (hi, hi, hi)))
+    pr.entropy_mesh = ([8, 8, 8], [lo, lo, lo], [hi, hi, hi])
+    pr.cell_temperatures = temps
+    return pr

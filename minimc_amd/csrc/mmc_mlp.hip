@@ -188,78 +188,214 @@ __global__ void KMlpPack(const float* P, int w0_off, int b0_off, int w1_off, int
 }
 
 // ---------------------------------------------------------------------------
-// The POD transport map (training targets).
-__device__ double BetaAtGrid(const DevWorld& W, const DTsl& t, int e_s, double T, double u) {
-  const double* Es = W.pool + t.Es_off;
-  int local;
-  const int p_i = PartitionOf(W.ipool + t.E_ends_off, t.n_bp, e_s, local);
-  const DTslPart P = W.parts[t.bp_begin + p_i];
-  const double* F = W.pool + P.F_off;
-  const int f_hi = UpperBoundArr(F, P.nF, u);
-  const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
-  const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-  const double b_lo = f_hi != 0 ? PartEvaluate(W, P, f_hi - 1, local, T)
-                                : -Es[e_s] / (kBoltzmann * T);
-  const double b_hi = f_hi != P.nF ? PartEvaluate(W, P, f_hi, local, T) : t.beta_cutoff;
-  return b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
+// The POD transport map (training targets): the quantile functions of the
+// distributions the POD sampler draws from.  SampleBeta picks the lower
+// (upper) tabulated energy with probability 1 - r (r), draws beta from that
+// grid's piecewise-linear inverse CDF and rejects beta < -E/kT; the beta
+// law is therefore a mixture of two truncated piecewise-linear laws, and
+// y1 comes from inverting its CDF by bisection.  SampleAlpha picks a beta
+// grid with the reference's interpolation factor, draws F uniformly between
+// the CDF values of the kinematic bounds and rescales alpha to the exact
+// bounds; the fraction y2 = (alpha - alpha_min) / (alpha_max - alpha_min)
+// follows the mixture of the two grids' fraction laws, inverted the same way.
+
+// POD coefficients of one (partition, x index, T): value(F_j) =
+// sum_r cdf[j][r] c[r] (linear in T between the tabulated temperatures).
+struct PodRow {
+  const double* cdf;
+  const double* F;
+  int nF, rank;
+  double c[16];
+};
+__device__ void MakeRow(const DevWorld& W, const DTslPart& P, int x_i, double T, PodRow& row) {
+  const double* Ts = W.pool + P.T_off;
+  const int cand = UpperBoundArr(Ts, P.nT, T);
+  const int t_hi = cand == P.nT ? cand - 1 : cand;
+  const int t_lo = t_hi == 0 ? t_hi : t_hi - 1;
+  const double w = t_hi == t_lo ? 0.0 : (T - Ts[t_lo]) / (Ts[t_hi] - Ts[t_lo]);
+  const double* S = W.pool + P.S_off;
+  const double* vh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
+  const double* vl = W.pool + P.xt_off + (x_i * P.nT + t_lo) * P.rank;
+  row.rank = P.rank < 16 ? P.rank : 16;
+  for (int r = 0; r < row.rank; r++) row.c[r] = S[r] * (vl[r] + w * (vh[r] - vl[r]));
+  row.cdf = W.pool + P.cdf_off;
+  row.F = W.pool + P.F_off;
+  row.nF = P.nF;
 }
-__device__ double AlphaFracAtGrid(const DevWorld& W, const DTsl& t, int b_i, int sgn, double b,
-                                  double E, double T, double u) {
-  const double* betas = W.pool + t.betas_off;
-  double b_s = sgn * betas[b_i];
-  if (E + b_s * kBoltzmann * T < 0) b_s = b;
-  const double sqrt_E = sqrt(E);
-  const double root = sqrt(E + b_s * kBoltzmann * T);
-  const double akT = t.awr * kBoltzmann * T;
-  const double a_min = (sqrt_E - root) * (sqrt_E - root) / akT;
-  const double a_max = (sqrt_E + root) * (sqrt_E + root) / akT;
-  int local;
-  const int p_i = PartitionOf(W.ipool + t.beta_ends_off, t.n_ap, b_i, local);
-  const DTslPart P = W.parts[t.ap_begin + p_i];
-  const double* F = W.pool + P.F_off;
-  const double F_min = FindCdf(W, P, local, T, a_min, t.alpha_cutoff);
-  const double F_max = FindCdf(W, P, local, T, a_max, t.alpha_cutoff);
-  const double Fq = F_min + u * (F_max - F_min);
-  const int hi = UpperBoundArr(F, P.nF, Fq);
-  const double F_lo = hi != 0 ? F[hi - 1] : 0;
-  const double F_hi = hi != P.nF ? F[hi] : 1;
-  const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
-  const double a_hi = hi != P.nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
-  const double a = a_lo + (Fq - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
-  const double fr = (a - a_min) / (a_max - a_min);
-  return fr < 0 ? 0 : (fr > 1 ? 1 : fr);
+__device__ __forceinline__ double RowValue(const PodRow& row, int j) {
+  const double* u = row.cdf + j * row.rank;
+  double v = 0;
+  for (int r = 0; r < row.rank; r++) v += u[r] * row.c[r];
+  return v;
 }
-__device__ void PodMap(const DevWorld& W, const DTsl& t, double E, double T, double u1,
-                       double u2, double& y1, double& y2) {
+// CDF of the piecewise-linear law with knots (F_j, value_j), end points
+// (0, lo_end) and (1, hi_end), evaluated at v.
+__device__ double RowCdf(const PodRow& row, double lo_end, double hi_end, double v) {
+  int first = 0, len = row.nF;
+  while (len > 0) {
+    const int half = len >> 1;
+    const int mid = first + half;
+    if (v < RowValue(row, mid)) {
+      len = half;
+    } else {
+      first = mid + 1;
+      len = len - half - 1;
+    }
+  }
+  const int hi = first;
+  const double F_lo = hi != 0 ? row.F[hi - 1] : 0.0;
+  const double F_hi = hi != row.nF ? row.F[hi] : 1.0;
+  const double a = hi != 0 ? RowValue(row, hi - 1) : lo_end;
+  const double b = hi != row.nF ? RowValue(row, hi) : hi_end;
+  if (!(b > a)) return v < a ? F_lo : F_hi;
+  double c = F_lo + (v - a) * (F_hi - F_lo) / (b - a);
+  return c < 0 ? 0 : (c > 1 ? 1 : c);
+}
+
+// Measure of F in [Fa, Fb] with phi(F) <= v, phi the piecewise-linear map
+// through (0, lo_end), (F_j, value_j), (1, hi_end).  Exact also where the
+// POD reconstruction makes phi non-monotone (then the sampler's law is not
+// the inverse of phi, but this measure).
+__device__ double RowMeasure(const PodRow& row, double lo_end, double hi_end, double v,
+                             double Fa, double Fb) {
+  double f0 = 0.0, a = lo_end, m = 0.0;
+  for (int j = 0; j <= row.nF; j++) {
+    const double f1 = j < row.nF ? row.F[j] : 1.0;
+    const double b = j < row.nF ? RowValue(row, j) : hi_end;
+    const double lo = f0 > Fa ? f0 : Fa, hi = f1 < Fb ? f1 : Fb;
+    if (hi > lo && f1 > f0) {
+      // phi on [lo, hi] is linear between pa and pb
+      const double pa = a + (b - a) * (lo - f0) / (f1 - f0);
+      const double pb = a + (b - a) * (hi - f0) / (f1 - f0);
+      if (pa <= v && pb <= v) {
+        m += hi - lo;
+      } else if (pa <= v || pb <= v) {
+        const double tcross = (v - pa) / (pb - pa);
+        m += (pa <= v ? tcross : 1.0 - tcross) * (hi - lo);
+      }
+    }
+    f0 = f1;
+    a = b;
+  }
+  return m;
+}
+
+__device__ double PodMapBeta(const DevWorld& W, const DTsl& t, double E, double T, double u) {
   const double* Es = W.pool + t.Es_off;
   int e_hi = UpperBoundArr(Es, t.nEs, E);
   if (e_hi >= t.nEs) e_hi = t.nEs - 1;
-  double b;
+  int g[2];
+  double wgt[2];
+  int ng = 0;
   if (e_hi == 0) {
-    b = BetaAtGrid(W, t, 0, T, u1);
+    g[0] = 0; wgt[0] = 1; ng = 1;
   } else {
     const double r = (E - Es[e_hi - 1]) / (Es[e_hi] - Es[e_hi - 1]);
-    b = (1 - r) * BetaAtGrid(W, t, e_hi - 1, T, u1) + r * BetaAtGrid(W, t, e_hi, T, u1);
+    g[0] = e_hi - 1; wgt[0] = 1 - r;
+    g[1] = e_hi; wgt[1] = r;
+    ng = 2;
   }
-  const double b_min = -E / (kBoltzmann * T);
-  if (b < b_min) b = b_min;
-  if (b > t.beta_cutoff) b = t.beta_cutoff;
+  const double kT = kBoltzmann * T;
+  const double b_min = -E / kT;
+  PodRow rows[2];
+  double lo_end[2], c0[2];
+  for (int q = 0; q < ng; q++) {
+    int local;
+    const int p_i = PartitionOf(W.ipool + t.E_ends_off, t.n_bp, g[q], local);
+    MakeRow(W, W.parts[t.bp_begin + p_i], local, T, rows[q]);
+    lo_end[q] = -Es[g[q]] / kT;
+    c0[q] = RowMeasure(rows[q], lo_end[q], t.beta_cutoff, b_min, 0.0, 1.0);  // rejected
+  }
+  auto mix = [&](double b) {
+    double m = 0;
+    for (int q = 0; q < ng; q++) {
+      const double c = RowMeasure(rows[q], lo_end[q], t.beta_cutoff, b, 0.0, 1.0);
+      const double d = 1 - c0[q];
+      m += wgt[q] * (d > 0 ? (c - c0[q]) / d : 1.0);
+    }
+    return m;
+  };
+  double lo = b_min, hi = t.beta_cutoff;
+  for (int it = 0; it < 34; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mix(mid) < u) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+__device__ double PodMapAlphaFrac(const DevWorld& W, const DTsl& t, double b, double E,
+                                  double T, double u) {
   const double* betas = W.pool + t.betas_off;
+  const double kT = kBoltzmann * T;
   const double ab = fabs(b);
   const int sgn = (0 < b) - (b < 0);
   int b_hi = UpperBoundArr(betas, t.nbetas, ab);
-  double fr;
+  if (b_hi >= t.nbetas) b_hi = t.nbetas - 1;
+  // grid choice of SampleAlpha (ThermalScattering.cpp:355-367, quirk Q2)
+  const bool snap_lower = (sgn == 1 && t.beta_cutoff <= betas[b_hi]) ||
+                          (sgn == -1 && -betas[b_hi] < -E / kT);
+  const bool snap_min = b_hi == 0;
+  double r = snap_lower ? 0
+             : snap_min ? 1
+                        : (ab - betas[b_hi - 1] / (betas[b_hi] - betas[b_hi - 1]));
+  r = r < 0 ? 0 : (r > 1 ? 1 : r);
+  int g[2];
+  double wgt[2];
+  int ng = 0;
   if (b_hi == 0) {
-    fr = AlphaFracAtGrid(W, t, 0, sgn, b, E, T, u2);
-  } else if (b_hi >= t.nbetas) {
-    fr = AlphaFracAtGrid(W, t, t.nbetas - 1, sgn, b, E, T, u2);
+    g[0] = 0; wgt[0] = 1; ng = 1;
   } else {
-    const double r = (ab - betas[b_hi - 1]) / (betas[b_hi] - betas[b_hi - 1]);
-    fr = (1 - r) * AlphaFracAtGrid(W, t, b_hi - 1, sgn, b, E, T, u2) +
-         r * AlphaFracAtGrid(W, t, b_hi, sgn, b, E, T, u2);
+    g[0] = b_hi - 1; wgt[0] = 1 - r;
+    g[1] = b_hi; wgt[1] = r;
+    ng = 2;
   }
+  const double sqrt_E = sqrt(E);
+  const double akT = t.awr * kT;
+  PodRow rows[2];
+  double amin[2], amax[2], Fmin[2], Fmax[2], m0[2], m1[2];
+  for (int q = 0; q < ng; q++) {
+    double b_s = sgn * betas[g[q]];
+    if (E + b_s * kT < 0) b_s = b;
+    const double root = sqrt(E + b_s * kT);
+    amin[q] = (sqrt_E - root) * (sqrt_E - root) / akT;
+    amax[q] = (sqrt_E + root) * (sqrt_E + root) / akT;
+    int local;
+    const int p_i = PartitionOf(W.ipool + t.beta_ends_off, t.n_ap, g[q], local);
+    MakeRow(W, W.parts[t.ap_begin + p_i], local, T, rows[q]);
+    // find_cdf of the reference (monotone search) fixes the F window ...
+    Fmin[q] = RowCdf(rows[q], 0.0, t.alpha_cutoff, amin[q]);
+    Fmax[q] = RowCdf(rows[q], 0.0, t.alpha_cutoff, amax[q]);
+    // ... and values outside (a_min, a_max) are rejected (resampled)
+    m0[q] = RowMeasure(rows[q], 0.0, t.alpha_cutoff, amin[q], Fmin[q], Fmax[q]);
+    m1[q] = RowMeasure(rows[q], 0.0, t.alpha_cutoff, amax[q], Fmin[q], Fmax[q]);
+  }
+  auto mix = [&](double f) {
+    double m = 0;
+    for (int q = 0; q < ng; q++) {
+      const double d = m1[q] - m0[q];
+      if (!(d > 0) || !(amax[q] > amin[q])) {
+        m += wgt[q] * (f >= 0.5 ? 1.0 : 0.0);
+        continue;
+      }
+      const double c = RowMeasure(rows[q], 0.0, t.alpha_cutoff,
+                                  amin[q] + f * (amax[q] - amin[q]), Fmin[q], Fmax[q]);
+      m += wgt[q] * (c - m0[q]) / d;
+    }
+    return m;
+  };
+  double lo = 0, hi = 1;
+  for (int it = 0; it < 26; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mix(mid) < u) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+__device__ void PodMap(const DevWorld& W, const DTsl& t, double E, double T, double u1,
+                       double u2, double& y1, double& y2) {
+  const double b = PodMapBeta(W, t, E, T, u1);
   y1 = b * (kBoltzmann * T) / (E + kBoltzmann * T);
-  y2 = fr;
+  y2 = PodMapAlphaFrac(W, t, b, E, T, u2);
 }
 // (y1, y2) -> (alpha, beta) in the trained parametrisation.
 __device__ __forceinline__ void OutputsToAlphaBeta(double y1, double y2, double E, double T,
@@ -389,12 +525,16 @@ static void LaunchFused(const MlpState& m, const float* x, float* y, const uint3
 }
 
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
-               uint32_t grid, hipStream_t s) {
+               uint32_t grid, hipStream_t s, void (*mark)(void*, int), void* mark_arg) {
   if (!m.ready || !m.xin) return 1;
+  mark(mark_arg, MMC_K_TSL);
   hipLaunchKernelGGL(KTslGather, dim3(grid), dim3(256), 0, s, W, R, m, iter);
   // the fused kernel reads the queue length itself; 256 blocks = 1 per CU
+  mark(mark_arg, MMC_K_MLP);
   LaunchFused(m, m.xin, m.yout, Ring(R, iter, kT), 0, 256, s);
+  mark(mark_arg, MMC_K_TSL);
   hipLaunchKernelGGL(KTslApply, dim3(grid), dim3(256), 0, s, W, R, m, iter, qin);
+  mark(mark_arg, -1);
   return 0;
 }
 
@@ -412,8 +552,12 @@ __global__ void __launch_bounds__(256)
   __shared__ float Bs[16][64 + 4];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  // split K over gridDim.z (partial sums added atomically when > 1)
+  const int kc = (K + gridDim.z - 1) / gridDim.z;
+  const int k_begin = blockIdx.z * kc;
+  const int k_end = min(K, k_begin + kc);
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int k0 = k_begin; k0 < k_end; k0 += 16) {
     for (int e = threadIdx.x; e < 64 * 16; e += 256) {
       int mm, kk;
       if (TA) {
@@ -425,7 +569,7 @@ __global__ void __launch_bounds__(256)
       }
       const int gm = m0 + mm, gk = k0 + kk;
       float v = 0.0f;
-      if (gm < M && gk < K) v = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
+      if (gm < M && gk < k_end) v = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
       As[kk][mm] = v;
       int nn, kb;
       if (TB) {
@@ -437,7 +581,7 @@ __global__ void __launch_bounds__(256)
       }
       const int gn = n0 + nn, gk2 = k0 + kb;
       float w = 0.0f;
-      if (gn < N && gk2 < K) w = TB ? Bm[size_t(gn) * ldb + gk2] : Bm[size_t(gk2) * ldb + gn];
+      if (gn < N && gk2 < k_end) w = TB ? Bm[size_t(gn) * ldb + gk2] : Bm[size_t(gk2) * ldb + gn];
       Bs[kb][nn] = w;
     }
     __syncthreads();
@@ -464,6 +608,10 @@ __global__ void __launch_bounds__(256)
       const int gn = n0 + tx * 4 + jj;
       if (gn >= N) continue;
       float v = acc[i][jj];
+      if (gridDim.z > 1) {
+        atomicAdd(C + size_t(gm) * ldc + gn, v);
+        continue;
+      }
       if (bias) v += bias[gn];
       C[size_t(gm) * ldc + gn] = v;
       if (H) H[size_t(gm) * ldc + gn] = v / (1.0f + expf(-v));
@@ -473,8 +621,9 @@ __global__ void __launch_bounds__(256)
 
 template <bool TA, bool TB>
 static void Gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B,
-                 int ldb, float* C, int ldc, const float* bias = nullptr, float* H = nullptr) {
-  dim3 grid((N + 63) / 64, (M + 63) / 64);
+                 int ldb, float* C, int ldc, const float* bias = nullptr, float* H = nullptr,
+                 int split = 1) {
+  dim3 grid((N + 63) / 64, (M + 63) / 64, split);
   hipLaunchKernelGGL((KGemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc,
                      bias, H);
 }
@@ -714,8 +863,12 @@ static int ForwardBackward(mmc_ctx* ctx, uint64_t rows, const float* x, const fl
   for (int l = L - 1; l >= 0; l--) {
     const int nout = m.widths[l + 1], nin = m.widths[l];
     const float* in = l == 0 ? x : H[l - 1];
-    // dW_l = dZ^T in
-    Gemm<true, false>(s, nout, nin, R, dZ, nout, in, nin, m.grads + m.w_off[l], nin);
+    // dW_l = dZ^T in, K = rows split over blocks (grads were zeroed)
+    const int tiles = ((nout + 63) / 64) * ((nin + 63) / 64);
+    int split = std::max(1, std::min(64, 512 / tiles));
+    split = std::min(split, std::max(1, R / 256));
+    Gemm<true, false>(s, nout, nin, R, dZ, nout, in, nin, m.grads + m.w_off[l], nin, nullptr,
+                      nullptr, split);
     hipLaunchKernelGGL(KColSum, dim3((nout + 63) / 64, (R + 1023) / 1024), dim3(256), 0, s, dZ,
                        R, nout, m.grads + m.b_off[l]);
     if (l > 0) {

@@ -42,7 +42,9 @@ namespace mmc {
 // One thermal-scattering step of the tracker with the surrogate: gathers
 // (log E, T, u1, u2) of the TSL queue, runs the network, applies the
 // scatter, and sends out-of-box inputs to the POD sampler.
+// `mark(k)` is called before each launch with the profiling kind of the
+// next kernel (MMC_K_TSL or MMC_K_MLP) and with -1 after the last one.
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
-               uint32_t grid, hipStream_t stream);
+               uint32_t grid, hipStream_t stream, void (*mark)(void*, int), void* mark_arg);
 
 }  // namespace mmc

@@ -704,6 +704,14 @@ static void ProfEnd(mmc_ctx* ctx) {
   hipEventRecord(ctx->ev_pool[ctx->ev_used + 1], ctx->stream);
   ctx->ev_used += 2;
 }
+// Bracket helper for launch sequences owned by other files: kind >= 0 closes
+// any open bracket and opens one of that kind, -1 closes.
+static void ProfMark(void* p, int kind) {
+  mmc_ctx* ctx = static_cast<mmc_ctx*>(p);
+  if (!ctx->prof) return;
+  if (ctx->ev_kind.size() * 2 > ctx->ev_used) ProfEnd(ctx);
+  if (kind >= 0) ProfBegin(ctx, kind);
+}
 // After a stream synchronisation: fold the recorded pairs into the totals.
 static void ProfCollect(mmc_ctx* ctx) {
   if (!ctx->prof) return;
@@ -733,15 +741,16 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
                      iter, qin);
   ProfEnd(ctx);
   if (ctx->has_tsl) {
-    ProfBegin(ctx, MMC_K_TSL);
     if (ctx->opt.tsl_mode == MMC_TSL_SURROGATE && ctx->mlp.ready) {
-      const int rc = MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, grid, ctx->stream);
+      const int rc = MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, grid, ctx->stream,
+                                ProfMark, ctx);
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
     } else {
+      ProfBegin(ctx, MMC_K_TSL);
       hipLaunchKernelGGL(KTslPod, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                          iter, qin);
+      ProfEnd(ctx);
     }
-    ProfEnd(ctx);
   }
   return MMC_OK;
 }

@@ -75,23 +75,54 @@ def test_fused_inference_matches_oracle(env):
     assert np.all(err.mean(0) <= 0.01 * spread), (err.mean(0), spread)
 
 
-def test_pod_map_reproduces_tables(env):
+def _measure(F, vals, lo_end, hi_end, v):
+    """Measure of F in [0, 1] with phi(F) <= v, phi piecewise linear through
+    (0, lo_end), (F_j, vals_j), (1, hi_end)."""
+    f = np.concatenate([[0.0], F, [1.0]])
+    p = np.concatenate([[lo_end], vals, [hi_end]])
+    m = 0.0
+    for a, b, f0, f1 in zip(p[:-1], p[1:], f[:-1], f[1:]):
+        if a <= v and b <= v:
+            m += f1 - f0
+        elif a <= v or b <= v:
+            tc = (v - a) / (b - a)
+            m += (tc if a <= v else 1 - tc) * (f1 - f0)
+    return m
+
+
+def test_pod_map_is_the_sampler_quantile(env):
+    """At a tabulated energy the POD sampler draws F uniformly, maps it
+    through the table's piecewise-linear inverse CDF and rejects
+    beta < -E/kT (ThermalScattering.cpp:302-334): the map must return the
+    quantile of that truncated law (computed here with brentq)."""
+    from scipy.optimize import brentq
     pr, t, s = env
     tab = configs.tsl_tables()
     o = Oracle(pr)
-    part = tab["beta_partitions"][1]
+    parts = tab["beta_partitions"]
     Ts = tab["total_T_axis"]
-    rows = []
-    for (i, j, k) in ((3, 50, 2), (10, 120, 10), (20, 180, 20)):
-        E = part["x"][i]
-        rows.append((np.log(E), Ts[k], part["F"][j], 0.5, i, j, k))
-    x = np.array([r[:4] for r in rows], dtype=np.float32)
+    rows, cases = [], []
+    for (pi, i, k) in ((1, 3, 2), (1, 10, 10), (1, 20, 20), (0, 30, 5), (0, 60, 15)):
+        E = parts[pi]["x"][i]
+        for u in (0.05, 0.3, 0.5, 0.8, 0.97):
+            rows.append((np.log(E), Ts[k], u, 0.5))
+            cases.append((pi, i, k, u))
+    x = np.array(rows, dtype=np.float32)
     y = s.pod_map(t.device_array(x), len(rows)).numpy()
-    for q, (_, T, F, _, i, j, k) in enumerate(rows):
+    for q, (pi, i, k, u) in enumerate(cases):
         E = float(np.exp(np.float64(x[q, 0])))
-        beta = o.tsl_evaluate(0, 1, j, i, T)
-        want = beta * K * T / (E + K * T)
-        assert y[q, 0] == pytest.approx(want, rel=2e-4, abs=1e-6)
+        T = float(x[q, 1])
+        kT = K * T
+        Eg = parts[pi]["x"][i]
+        F = parts[pi]["F"]
+        vals = np.array([o.tsl_evaluate(0, pi, j, i, T) for j in range(len(F))])
+        lo_end = -Eg / kT
+        c0 = _measure(F, vals, lo_end, tab["beta_cutoff"], -E / kT)
+        uu = float(x[q, 2])
+        b = brentq(lambda v: (_measure(F, vals, lo_end, tab["beta_cutoff"], v) - c0)
+                   / (1 - c0) - uu, -E / kT, tab["beta_cutoff"], xtol=1e-12)
+        want = b * kT / (E + kT)
+        assert y[q, 0] == pytest.approx(want, rel=1e-4, abs=1e-6), (pi, i, k, u)
     # monotone in u1, fraction in [0, 1]
     n = 2000
     x = np.zeros((n, 4), dtype=np.float32)
@@ -112,7 +143,8 @@ def _w1_rel(a, b):
 def test_trained_surrogate_matches_pod_sampler(env):
     pr, t, s = env
     losses = s.train(n_samples=1 << 21, batch=8192, epochs=4, lr=2e-3, seed=5)
-    assert losses[-1] < 0.2 * losses[0]
+    # normalised MSE: the first value is already after 1/8 epoch
+    assert losses[-1] < 0.5 * losses[0] and losses[-1] < 0.01, losses
     n = 200000
     for E_eV, T in ((0.02, 350.0), (0.6, 800.0), (3.0, 1150.0)):
         E = t.device_array(np.full(n, E_eV * 1e-6))
@@ -128,5 +160,7 @@ def test_trained_surrogate_matches_pod_sampler(env):
         ap, bp, am, bm = a1.numpy(), b1.numpy(), a2.numpy(), b2.numpy()
         assert np.all(np.isfinite(am)) and np.all(np.isfinite(bm))
         # a small training run: a loose bound (the full run is judged in bench)
+        print("W1 rel beta %.4f alpha %.4f at E=%g eV T=%g" % (
+            _w1_rel(bm, bp), _w1_rel(am, ap), E_eV, T))
         assert _w1_rel(bm, bp) < 0.15, (E_eV, T, _w1_rel(bm, bp))
         assert _w1_rel(am, ap) < 0.15, (E_eV, T, _w1_rel(am, ap))
