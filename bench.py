@@ -1,25 +1,39 @@
 """Benchmark of the MI355X particle-history tracker.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c2] [--no-cpu]
+                [--train-epochs E]
 
-One step is one pass of the hot path over one batch of synthetic input:
-  c2  continuous-energy fixed-source slab: one batch of 1e6 histories
-      (the full workload is 1e7 histories = 10 steps)
+One step is one pass of the hot path over one batch of synthetic input.
+
+  c5 (default, the headline workload) multi-zone continuous-temperature
+     k-eigenvalue problem: 60 cm box of 27 cubes (fuel, bound-hydrogen
+     moderator, structure), per-cube temperature 300-1200 K, 1e6 histories
+     per cycle.  One step = one active cycle; the warmup steps are the
+     inactive cycles (defaults: 20 inactive + 100 active, the full problem).
+     The problem is run twice: with the POD thermal-scattering sampler and
+     with the learned sampler (BF16-MFMA network, POD outside its box).  The
+     figure of merit of each run is 1/(sigma_rel(k)^2 T) over the active
+     cycles; `value` is the surrogate run's histories/s.
+  c2 continuous-energy fixed-source slab: one step = one batch of 1e6
+     histories (the full workload is 1e7 histories = 10 steps).
+
 Prints one JSON line (rank 0).  The timed region starts with all inputs in
 HBM and is bracketed by device synchronisations.  Kernel durations come from
 HIP events recorded by the library on its own stream.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 
 # Algorithmic bytes per event of each kernel (particle state read/written
 # once plus the table entries strictly needed; search probes and queue
@@ -32,19 +46,28 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #            write dir 24 + E 8 + rng 4 + slot 4 = 112 + tables
 #   cross:   read pos 24 + dir 24 + E 8 + surf 4 + cell 4 + hist 4 + slot 4,
 #            write cell 4 + slot 4 + tally record 16 = 96
+#   tsl:     read dir 24 + E 8 + cell 4 + rng 4 + slot 4, write dir 24 + E 8
+#            + rng 4 + slot 4 = 84 + four rank-8 POD reconstructions (two
+#            bracketing CDF rows each for beta and alpha; 8 coefficients +
+#            8 temperature-basis values = 128 B each)
+#   mlp:     FLOP per sample of the 4-256-256-256-2 forward pass
 BYTES = {
     "c2": {"advance": 108 + 32 + 2 * 8,        # total xs + 2 planes
            "collide": 112 + 3 * 32,            # total, capture, elastic
+           "cross": 96, "tsl": 0},
+    "c5": {"advance": 108 + 32 + 6 * 8,        # total xs + 6 cube faces
+           "collide": 112 + 3 * 32,            # total, reaction split, elastic
            "cross": 96,
-           "tsl": 0},
+           "tsl": 84 + 4 * 128},
 }
+MLP_FLOP_PER_SAMPLE = 2 * (4 * 256 + 256 * 256 + 256 * 256 + 256 * 2)
 
 
 def cpu_baseline(cfg, threads, target_s=12.0):
     """The oracle (CPU restatement, `threads` workers) on a bounded sample."""
     from oracle.oracle import Oracle
+    from minimc_amd import configs
     if cfg == "c2":
-        from minimc_amd import configs
         pr = configs.c2_slab()
         o = Oracle(pr)
         n = 2000
@@ -59,7 +82,47 @@ def cpu_baseline(cfg, threads, target_s=12.0):
         return {"value": n2 / dt, "unit": "histories/s", "cores": threads,
                 "kind": "port",
                 "sample": "%d histories of the 1e7-history CE slab, %.1f s" % (n2, dt)}
+    if cfg == "c5":
+        # POD sampler (the reference's method); 2 untimed inactive cycles, then
+        # active cycles until about target_s of work is timed
+        pr = configs.c5_multizone()
+        n = 100_000
+        o = Oracle(pr)
+        o.set_entropy_mesh(*pr.entropy_mesh)
+        o.keig_init(n)
+        for _ in range(2):
+            o.run_cycle(active=False, threads=threads)
+        cycles, hist, dt = 0, 0, 0.0
+        while dt < target_s and cycles < 200:
+            t0 = time.perf_counter()
+            st = o.run_cycle(active=True, threads=threads)
+            dt += time.perf_counter() - t0
+            hist += n
+            cycles += 1
+        return {"value": hist / dt, "unit": "histories/s", "cores": threads,
+                "kind": "port", "k_collision_last": st["k_collision"],
+                "sample": "%d active cycles of %d histories of the multi-zone problem "
+                          "(POD sampler), %.1f s" % (cycles, n, dt)}
     raise ValueError(cfg)
+
+
+def _roofline(cfg, kt, counts):
+    """Roofline of the event kernel with the largest total time (the
+    history-based tail kernel is latency-bound by construction and is
+    reported beside, not here)."""
+    dom = max((k for k in kt if k != "tail"), key=lambda k: kt[k][0])
+    ms, launches = kt[dom]
+    if dom == "mlp":
+        achieved = counts["mlp"] * MLP_FLOP_PER_SAMPLE / (ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": dom, "achieved": achieved,
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                "flop_per_sample": MLP_FLOP_PER_SAMPLE,
+                "avg_launch_ms": ms / max(launches, 1)}
+    achieved = counts[dom] * BYTES[cfg][dom] / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_event": BYTES[cfg][dom], "avg_launch_ms": ms / max(launches, 1)}
 
 
 def run_c2(args):
@@ -86,58 +149,164 @@ def run_c2(args):
     leak = sum(t.tallies(e)[0].sum() for e in range(2))
     p = leak / n
     rel_var = (1 - p) / (p * n) if 0 < p < 1 else float("nan")
-    fom = 1.0 / (rel_var * dt)
     counts = {"advance": ev["advances"], "collide": ev["collisions"],
-              "cross": ev["crossings"], "tsl": ev["tsl_collisions"]}
-    dom = max(kt, key=lambda k: kt[k][0])
-    bytes_dom = counts[dom] * BYTES["c2"][dom]
-    ms_dom, launches = kt[dom]
-    achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
-    out = {
-        "metric": "FOM 1/(sigma^2 T) + histories/s, surrogate-TSL vs POD-TSL tracker, 1 MI355X",
-        "value": n / dt, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+              "cross": ev["crossings"], "tsl": ev["tsl_collisions"], "mlp": 0}
+    kt = {k: v for k, v in kt.items() if v[1] > 0}
+    return {
+        "value": n / dt, "unit": "histories/s", "ms_per_step": dt * 1e3 / args.steps,
+        "dtype": "f64",
         "config": {"workload": "c2: continuous-energy fixed-source slab, 60k-point "
                    "synthetic XS, 2 MeV point source, 1e6 histories per step",
                    "histories_per_step": batch},
-        "fom": fom, "fom_tally": "total leakage", "leakage_probability": p,
-        "collisions_per_s": ev["collisions"] / dt,
-        "events": counts,
-        "kernel_ms": {k: v[0] for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_event": BYTES["c2"][dom],
-                     "avg_launch_ms": ms_dom / max(launches, 1)},
+        "fom": 1.0 / (rel_var * dt), "fom_tally": "total leakage",
+        "leakage_probability": p, "collisions_per_s": ev["collisions"] / dt,
+        "events": counts, "kernel_ms": {k: v[0] for k, v in kt.items()},
+        "roofline": _roofline("c2", kt, counts),
     }
-    return out
+
+
+def _keig_leg(pr, tracker, warmup, steps):
+    """Inactive cycles (untimed), then `steps` timed active cycles."""
+    t = tracker
+    t.set_entropy_mesh(*pr.entropy_mesh)
+    t.keig_init(pr.histories)
+    for _ in range(warmup):
+        t.run_cycle(active=False)
+    c0 = t.counters()
+    t.set_profiling(True)
+    ks, ents = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = t.run_cycle(active=True)   # returns after a device synchronisation
+        ks.append(st["k_collision"])
+        ents.append(st["entropy"])
+    dt = time.perf_counter() - t0
+    kt = t.kernel_times()
+    t.set_profiling(False)
+    c1 = t.counters()
+    ev = {k: c1[k] - c0[k] for k in c1}
+    ks = np.array(ks)
+    k = float(ks.mean())
+    sig = float(ks.std(ddof=1) / np.sqrt(len(ks))) if len(ks) > 1 else float("nan")
+    rel = sig / k
+    return {"seconds": dt, "k": k, "sigma_k": sig, "fom": 1.0 / (rel * rel * dt),
+            "histories_per_s": steps * pr.histories / dt,
+            "collisions_per_s": ev["collisions"] / dt,
+            "entropy_last": ents[-1], "events": ev,
+            "kernel_ms": {n: v[0] for n, v in kt.items() if v[1] > 0},
+            "kernel_launches": {n: v[1] for n, v in kt.items() if v[1] > 0},
+            "_kt": kt}
+
+
+def tsl_batch_rates(tracker, n=1 << 22, reps=5, seed=4):
+    """Batch thermal-scattering sampling (2^22 collisions, E log-uniform on
+    1e-5..4 eV and T uniform on 300..1200 K): POD sampler vs learned sampler,
+    samples/s over `reps` synchronised calls each."""
+    rng = np.random.default_rng(seed)
+    E = tracker.device_array(np.exp(rng.uniform(np.log(1e-11), np.log(3.99e-6), n)))
+    T = tracker.device_array(rng.uniform(300.0, 1200.0, n))
+    r = tracker.device_array(rng.integers(1, 2**31 - 1, n).astype(np.uint32))
+    a = tracker.device_array(shape=n, dtype=np.float64)
+    b = tracker.device_array(shape=n, dtype=np.float64)
+    out = {}
+    for name, fn in (("pod", tracker.tsl_sample_pod), ("surrogate", tracker.tsl_sample_surrogate)):
+        fn(E, T, r, a, b)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(E, T, r, a, b)
+        out[name] = n * reps / (time.perf_counter() - t0)
+    for x in (E, T, r, a, b):
+        x.free()
+    return {"batch": n, "pod_samples_per_s": out["pod"],
+            "surrogate_samples_per_s": out["surrogate"],
+            "speedup": out["surrogate"] / out["pod"]}
+
+
+def run_c5(args):
+    from minimc_amd import abi, configs, surrogate
+    from minimc_amd.tracker import Tracker
+    pr = configs.c5_multizone()
+    n = pr.histories
+    # plain POD tracker
+    tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n)
+    pod = _keig_leg(pr, tp, args.warmup, args.steps)
+    tp.close()
+    # surrogate tracker: the network is trained once (outside every timed
+    # region) and cached in minimc_amd/_build
+    ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n)
+    t0 = time.perf_counter()
+    net_path = os.path.join(ROOT, "minimc_amd", "_build", "tsl_surrogate.npz")
+    trained_here = not os.path.exists(net_path)
+    surrogate.cached_network(ts, path=net_path, epochs=args.train_epochs, batch=16384,
+                             lr=3e-3)
+    t_net = time.perf_counter() - t0
+    sur = _keig_leg(pr, ts, args.warmup, args.steps)
+    batch = tsl_batch_rates(ts)
+    ts.close()
+    ev = sur["events"]
+    kt = sur.pop("_kt")
+    pod.pop("_kt")
+    counts = {"advance": ev["advances"], "collide": ev["collisions"],
+              "cross": ev["crossings"],
+              "tsl": ev["tsl_collisions"] - ev["tsl_fallbacks"],
+              "mlp": ev["tsl_collisions"] - ev["tsl_fallbacks"]}
+    kt = {k: v for k, v in kt.items() if v[1] > 0}
+    dk = abs(sur["k"] - pod["k"]) / np.hypot(sur["sigma_k"], pod["sigma_k"])
+    return {
+        "value": sur["histories_per_s"], "unit": "histories/s",
+        "ms_per_step": sur["seconds"] * 1e3 / args.steps, "dtype": "f64",
+        "config": {"workload": "c5: multi-zone continuous-temperature k-eigenvalue, "
+                   "60 cm box of 27 cubes (fuel / bound-H moderator / structure), "
+                   "per-cube T 300-1200 K, surrogate thermal scattering in the "
+                   "moderator; 1e6 histories per cycle, one step = one active cycle",
+                   "histories_per_cycle": n, "inactive_cycles": args.warmup,
+                   "active_cycles": args.steps},
+        "fom_ratio": sur["fom"] / pod["fom"],
+        "histories_per_s_ratio": sur["histories_per_s"] / pod["histories_per_s"],
+        "k_diff_sigma": float(dk),
+        "surrogate": sur, "pod": pod, "tsl_batch": batch,
+        "network": {"trained_here": trained_here, "seconds": t_net,
+                    "path": os.path.relpath(net_path, ROOT)},
+        "roofline": _roofline("c5", kt, counts),
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--config", default="c5", choices=("c5", "c2"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--train-epochs", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; "
               "the multi-GPU design (history sharding + one all-reduce of FP64 "
               "tallies) is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 100 if args.config == "c5" else 10
+    if args.warmup is None:
+        args.warmup = 20 if args.config == "c5" else 1
     rank = int(os.environ.get("RANK", "0"))
-    out = run_c2(args)
+    res = run_c5(args) if args.config == "c5" else run_c2(args)
+    out = {"metric": "FOM 1/(sigma^2 T) + histories/s, surrogate-TSL vs POD-TSL "
+                     "tracker, 1 MI355X",
+           "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
+           "data": "synthetic"}
+    out.update(res)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.config, threads)
     else:
         out["cpu_baseline"] = None
     if rank == 0:
-        print(json.dumps(out))
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -413,7 +413,8 @@ enum mmc_kernel_kind {
   MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling: the
                         POD sampler, or the surrogate's gather + apply */
   MMC_K_MLP = 4,     /* the surrogate's fused BF16-MFMA network        */
-  MMC_K_N = 5
+  MMC_K_TAIL = 5,    /* history-based completion of the last particles */
+  MMC_K_N = 6
 };
 int mmc_set_profiling(mmc_ctx* ctx, int enable);
 /* total_ms[k] and launches[k] for k < MMC_K_N (accumulated since enable). */
@@ -445,14 +446,16 @@ int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
 /* ---- thermal-scattering surrogate (no counterpart in the reference) -----
  * MLP (log E, T, u1, u2) -> (alpha, beta), widths w[0..n_layers], SiLU
  * between layers.  Parameters are FP32 masters; inference runs BF16 MFMA
- * with FP32 accumulation.  Inputs outside [E <= e_max, t_lo <= T <= t_hi]
- * fall back to the POD sampler. */
+ * with FP32 accumulation.  Inputs outside [e_min <= E <= e_max,
+ * t_lo <= T <= t_hi], and network outputs outside the kinematic support
+ * (E' < 0), fall back to the POD sampler. */
 typedef struct mmc_mlp_desc {
   int32_t n_layers; /* number of linear layers                         */
   int32_t widths[8];
   float in_shift[4], in_scale[4];   /* x_norm = (x - shift) * scale        */
   float out_shift[2], out_scale[2]; /* y = y_norm * scale + shift          */
   double e_max, t_lo, t_hi;         /* trained box                         */
+  double e_min;
 } mmc_mlp_desc;
 /* params: host FP32, layer l: W_l [widths[l+1] x widths[l]] row-major then
  * b_l [widths[l+1]], layers consecutive. */

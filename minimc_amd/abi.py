@@ -194,7 +194,8 @@ class MlpDesc(C.Structure):
     _fields_ = [("n_layers", C.c_int32), ("widths", C.c_int32 * 8),
                 ("in_shift", C.c_float * 4), ("in_scale", C.c_float * 4),
                 ("out_shift", C.c_float * 2), ("out_scale", C.c_float * 2),
-                ("e_max", C.c_double), ("t_lo", C.c_double), ("t_hi", C.c_double)]
+                ("e_max", C.c_double), ("t_lo", C.c_double), ("t_hi", C.c_double),
+                ("e_min", C.c_double)]
 
 
 class TrainOpts(C.Structure):

@@ -262,7 +262,7 @@ def c5_moderator_nuclide(temperature=ROOM_TEMPERATURE):
         dict(kind=abi.MMC_RX_SCATTER, xs=(E, el), temperature=temperature, tsl=0)])
 
 
-def c5_multizone(histories=1_000_000, fuel_aden=0.02, mod_aden=H_ADEN, refl_aden=0.08,
+def c5_multizone(histories=1_000_000, fuel_aden=0.005, mod_aden=H_ADEN, refl_aden=0.08,
                  seed=20261202, tables=None):
     from . import synth
     t = tables if tables is not None else tsl_tables()

@@ -223,84 +223,136 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
   if (ok) R.Q[0][iq] = i;
 }
 
+#endif  // MMC_TRACKER_KERNELS
+
+// Copies the first grid's bucket hash into LDS (block-wide; every thread of
+// the block must call it).  Returns the pointer lookups use, or nullptr when
+// there is no hash to stage.
+__device__ __forceinline__ const int* StageHash(const DevWorld& W, int* lds_hash) {
+  if (W.energy_mode != 1 || W.grids[0].n_hash <= 0) return nullptr;
+  const DGrid g0 = W.grids[0];
+  const int* src = W.ipool + g0.hash_off;
+  // several loads in flight per thread before the LDS stores
+  constexpr int kU = 8;
+  for (int k0 = threadIdx.x; k0 <= g0.n_hash; k0 += kU * blockDim.x) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      v[u] = k <= g0.n_hash ? src[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      if (k <= g0.n_hash) lds_hash[k] = v[u];
+    }
+  }
+  __syncthreads();
+  return lds_hash;
+}
+
 // ---- advance: XS lookup + distance to collision + CSG distance ------------
 // SurfaceTracking/CellDeltaTracking::Transport loop body up to the branch
-// (TransportMethod.cpp:57-66, :93-118).
+// (TransportMethod.cpp:57-66, :93-118).  Next event of particle p: a surface
+// crossing (to_x), a real collision (to_c) or, with delta tracking, a
+// virtual collision (to_q: back to the live queue).
+struct AdvanceOut {
+  bool to_x, to_c, to_q;
+};
+__device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
+                                                   uint32_t p, const int* hash) {
+  AdvanceOut o{false, false, false};
+  const bool delta = W.tracking == 1;
+  V3 pos = LoadPos(R.B, p);
+  const V3 dir = LoadDir(R.B, p);
+  const double E = R.B.E[p];
+  uint32_t g = R.B.rng[p];
+  const int cell = R.B.cell[p];
+  const DCell c = W.cells[cell];
+  const DField fld = W.fields[c.field];
+  const double aden = W.mats[c.material].aden;
+  const double T = delta ? (fld.kind == 0 ? fld.c : fld.upper) : FieldAt(fld, pos);
+  const double sigma = MaterialTotal(W, c.material, E, T, delta, hash);
+  const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
+  int s;
+  const double d_surf = NearestSurface(W, cell, pos, dir, s);
+  const bool cross = delta ? (d_surf < d_coll) : !(d_coll < d_surf);
+  double d;
+  if (cross) {
+    d = d_surf + kNudge;
+    R.B.surf[p] = s;
+    o.to_x = true;
+  } else {
+    bool real = true;
+    if (delta) {
+      const double tot = MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash);
+      real = Canonical(g) < tot / sigma;  // pre-flight position (quirk Q6)
+    }
+    d = d_coll;
+    o.to_c = real;
+    o.to_q = !real;
+  }
+  if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E);
+  pos = Add(pos, Mul(dir, d));
+  R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
+  R.B.rng[p] = g;
+  const uint32_t st = R.B.steps[p] + 1;
+  R.B.steps[p] = st;
+  if (st > R.max_steps) {
+    atomicOr(R.misc, kErrSteps);
+    o.to_x = o.to_c = o.to_q = false;
+  }
+  return o;
+}
+
+#ifdef MMC_TRACKER_KERNELS
 __global__ void __launch_bounds__(256) KAdvance(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
-  const int* hash = nullptr;
-  if (W.energy_mode == 1 && W.grids[0].n_hash > 0) {
-    const DGrid g0 = W.grids[0];
-    for (int k = threadIdx.x; k <= g0.n_hash; k += blockDim.x)
-      lds_hash[k] = W.ipool[g0.hash_off + k];
-    __syncthreads();
-    hash = lds_hash;
-  }
   if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
   const uint32_t n = *Ring(R, iter - 1, kQ);
+  if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
+  const int* hash = StageHash(W, lds_hash);
   const uint32_t* Q = R.Q[qin];
-  const bool delta = W.tracking == 1;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
-    bool to_x = false, to_c = false, to_q = false;
+    AdvanceOut o{false, false, false};
     uint32_t p = 0;
     if (valid) {
       p = Q[q];
-      V3 pos = LoadPos(R.B, p);
-      const V3 dir = LoadDir(R.B, p);
-      const double E = R.B.E[p];
-      uint32_t g = R.B.rng[p];
-      const int cell = R.B.cell[p];
-      const DCell c = W.cells[cell];
-      const DField fld = W.fields[c.field];
-      const double aden = W.mats[c.material].aden;
-      const double T = delta ? (fld.kind == 0 ? fld.c : fld.upper) : FieldAt(fld, pos);
-      const double sigma = MaterialTotal(W, c.material, E, T, delta, hash);
-      const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
-      int s;
-      const double d_surf = NearestSurface(W, cell, pos, dir, s);
-      const bool cross = delta ? (d_surf < d_coll) : !(d_coll < d_surf);
-      double d;
-      if (cross) {
-        d = d_surf + kNudge;
-        R.B.surf[p] = s;
-        to_x = true;
-      } else {
-        bool real = true;
-        if (delta) {
-          const double tot =
-              MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash);
-          real = Canonical(g) < tot / sigma;  // pre-flight position (quirk Q6)
-        }
-        d = d_coll;
-        to_c = real;
-        to_q = !real;
-      }
-      if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E);
-      pos = Add(pos, Mul(dir, d));
-      R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
-      R.B.rng[p] = g;
-      const uint32_t st = R.B.steps[p] + 1;
-      R.B.steps[p] = st;
-      if (st > R.max_steps) {
-        atomicOr(R.misc, kErrSteps);
-        to_x = to_c = to_q = false;
-      }
+      o = AdvanceEvent(W, R, p, hash);
     }
     uint32_t tot;
-    const uint32_t ix = WavePush(Ring(R, iter, kX), to_x, tot);
-    if (to_x) R.QX[ix] = p;
-    const uint32_t ic = WavePush(Ring(R, iter, kC), to_c, tot);
-    if (to_c) R.QC[ic] = p;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), to_q, tot);
-    if (to_q) R.Q[qin ^ 1][iq] = p;
+    const uint32_t ix = WavePush(Ring(R, iter, kX), o.to_x, tot);
+    if (o.to_x) R.QX[ix] = p;
+    const uint32_t ic = WavePush(Ring(R, iter, kC), o.to_c, tot);
+    if (o.to_c) R.QC[ic] = p;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), o.to_q, tot);
+    if (o.to_q) R.Q[qin ^ 1][iq] = p;
     WaveCount(R.ev + kEvAdvance, valid);
-    WaveCount(R.ev + kEvVirtual, to_q);
+    WaveCount(R.ev + kEvVirtual, o.to_q);
   }
 }
+#endif  // MMC_TRACKER_KERNELS
 
 // ---- surface crossing (TransportMethod.cpp:67-73, :98-105) ----------------
+// Returns whether the particle is still alive (surface_cross vs leak).
+__device__ __forceinline__ bool CrossEvent(const DevWorld& W, const RunState& R, uint32_t p) {
+  const V3 pos = LoadPos(R.B, p);
+  const int surf = R.B.surf[p];
+  const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
+                       ? FindCell(W, pos)
+                       : FindCellAcross(W, pos, surf, 1 - FaceInside(W, R.B.cell[p], surf));
+  if (cell < 0) {
+    atomicOr(R.misc, kErrLost);
+    return false;
+  }
+  R.B.cell[p] = cell;
+  if (W.n_est_current) ScoreCurrent(W, R, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
+  return W.cells[cell].material >= 0;
+}
+
+#ifdef MMC_TRACKER_KERNELS
 __global__ void __launch_bounds__(256) KCross(DevWorld W, RunState R, int iter, int qin) {
   const uint32_t n = *Ring(R, iter, kX);
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -310,19 +362,7 @@ __global__ void __launch_bounds__(256) KCross(DevWorld W, RunState R, int iter, 
     uint32_t p = 0;
     if (valid) {
       p = R.QX[q];
-      const V3 pos = LoadPos(R.B, p);
-      const int surf = R.B.surf[p];
-      const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
-                           ? FindCell(W, pos)
-                           : FindCellAcross(W, pos, surf, 1 - FaceInside(W, R.B.cell[p], surf));
-      if (cell < 0) {
-        atomicOr(R.misc, kErrLost);
-      } else {
-        R.B.cell[p] = cell;
-        alive = W.cells[cell].material >= 0;  // surface_cross vs leak
-        if (W.n_est_current)
-          ScoreCurrent(W, R, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
-      }
+      alive = CrossEvent(W, R, p);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
@@ -567,18 +607,53 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
   }
 }
 
+// Real collision of particle p (TransportMethod.cpp:74-90 / :106-118 with
+// Particle::Scatter).  `alive`: back to the live queue; `to_t`: a bound
+// thermal scattering whose (alpha, beta) is sampled by the TSL step.
+// Children of a fixed-source fission go to live_q / live_cnt.
+__device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& R, uint32_t p,
+                                             const int* hash, uint32_t* live_q,
+                                             uint32_t* live_cnt, bool& alive, bool& to_t) {
+  alive = to_t = false;
+  const V3 pos = LoadPos(R.B, p);
+  V3 dir = LoadDir(R.B, p);
+  double E = R.B.E[p];
+  uint32_t g = R.B.rng[p];
+  const int cell = R.B.cell[p];
+  const DCell c = W.cells[cell];
+  const double T = FieldAt(W.fields[c.field], pos);
+  if (W.n_est_flux || R.keig) {
+    const double tot = MaterialTotal(W, c.material, E, T, false, hash);
+    if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
+    if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E) / tot;
+  }
+  double kabs = 0;
+  int tsl = -1;
+  const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, live_q, live_cnt, hash,
+                               R.keig ? &kabs : nullptr, &tsl);
+  if (o.children < 0) atomicOr(R.misc, kErrReaction);
+  if (R.keig) {
+    R.B.k_abs[p] += kabs;
+    if (o.event == MMC_EV_FISSION) R.fis_cnt[R.B.hist[p]] = uint32_t(o.children);
+  }
+  if (o.tsl) {
+    to_t = true;
+    R.B.tslT[p] = T;
+    R.B.tslI[p] = tsl;
+  } else {
+    alive = o.event == MMC_EV_SCATTER || o.event < 0;
+  }
+  R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+  R.B.E[p] = E;
+  R.B.rng[p] = g;
+}
+
 #ifdef MMC_TRACKER_KERNELS
 __global__ void __launch_bounds__(256) KCollide(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
-  const int* hash = nullptr;
-  if (W.energy_mode == 1 && W.grids[0].n_hash > 0) {
-    const DGrid g0 = W.grids[0];
-    for (int k = threadIdx.x; k <= g0.n_hash; k += blockDim.x)
-      lds_hash[k] = W.ipool[g0.hash_off + k];
-    __syncthreads();
-    hash = lds_hash;
-  }
   const uint32_t n = *Ring(R, iter, kC);
+  if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
+  const int* hash = StageHash(W, lds_hash);
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -586,37 +661,7 @@ __global__ void __launch_bounds__(256) KCollide(DevWorld W, RunState R, int iter
     uint32_t p = 0;
     if (valid) {
       p = R.QC[q];
-      const V3 pos = LoadPos(R.B, p);
-      V3 dir = LoadDir(R.B, p);
-      double E = R.B.E[p];
-      uint32_t g = R.B.rng[p];
-      const int cell = R.B.cell[p];
-      const DCell c = W.cells[cell];
-      const double T = FieldAt(W.fields[c.field], pos);
-      if (W.n_est_flux || R.keig) {
-        const double tot = MaterialTotal(W, c.material, E, T, false, hash);
-        if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
-        if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E) / tot;
-      }
-      double kabs = 0;
-      int tsl = -1;
-      const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, R.Q[qin ^ 1],
-                                   Ring(R, iter, kQ), hash, R.keig ? &kabs : nullptr, &tsl);
-      if (o.children < 0) atomicOr(R.misc, kErrReaction);
-      if (R.keig) {
-        R.B.k_abs[p] += kabs;
-        if (o.event == MMC_EV_FISSION) R.fis_cnt[R.B.hist[p]] = uint32_t(o.children);
-      }
-      if (o.tsl) {
-        to_t = true;
-        R.B.tslT[p] = T;
-        R.B.tslI[p] = tsl;
-      } else {
-        alive = o.event == MMC_EV_SCATTER || o.event < 0;
-      }
-      R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
-      R.B.E[p] = E;
-      R.B.rng[p] = g;
+      CollideEvent(W, R, p, hash, R.Q[qin ^ 1], Ring(R, iter, kQ), alive, to_t);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
@@ -645,6 +690,25 @@ __device__ __forceinline__ uint32_t TslScatterPod(const DevWorld& W, const DTsl&
   return 0;
 }
 
+// Thermal scattering of particle p with the POD sampler; false on a
+// sampling error (flagged in misc).
+__device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R, uint32_t p) {
+  V3 dir = LoadDir(R.B, p);
+  double E = R.B.E[p];
+  uint32_t g = R.B.rng[p];
+  double a = 0, b = 0;
+  const uint32_t e = TslScatterPod(W, W.tsl[R.B.tslI[p]], g, R.B.tslT[p], dir, E, a, b);
+  if (e) {
+    atomicOr(R.misc, e);
+    return false;
+  }
+  if (W.n_est_tsl) ScoreAlphaBeta(W, R, R.B.hist[p], a, b);
+  R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+  R.B.E[p] = E;
+  R.B.rng[p] = g;
+  return true;
+}
+
 #ifdef MMC_TRACKER_KERNELS
 __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
   const uint32_t n = *Ring(R, iter, kT);
@@ -655,25 +719,59 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
     uint32_t p = 0;
     if (valid) {
       p = R.QT[q];
-      V3 dir = LoadDir(R.B, p);
-      double E = R.B.E[p];
-      uint32_t g = R.B.rng[p];
-      double a = 0, b = 0;
-      const uint32_t e = TslScatterPod(W, W.tsl[R.B.tslI[p]], g, R.B.tslT[p], dir, E, a, b);
-      if (e) {
-        atomicOr(R.misc, e);
-      } else {
-        alive = true;
-        if (W.n_est_tsl) ScoreAlphaBeta(W, R, R.B.hist[p], a, b);
-        R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
-        R.B.E[p] = E;
-        R.B.rng[p] = g;
-      }
+      alive = TslPodEvent(W, R, p);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
     if (alive) R.Q[qin ^ 1][iq] = p;
     WaveCount(R.ev + kEvTsl, valid);
+  }
+}
+
+// ---- tail: history-based completion of the last live particles ------------
+// When few particles are left, one event per kernel launch leaves the GPU
+// idle between short launches; this kernel instead runs every remaining
+// particle to the end of its history in one thread, with the same event
+// functions in the same order (so the same random numbers and results).
+// Thermal scatterings use the POD sampler here.  Used only where a
+// collision cannot start a new particle (k-eigenvalue, or no fission).
+__global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, int qin) {
+  const uint32_t n = *Ring(R, iter - 1, kQ);
+  unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0, c_tail = 0;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t p = R.Q[qin][q];
+    c_tail++;
+    for (;;) {
+      const AdvanceOut o = AdvanceEvent(W, R, p, nullptr);
+      c_adv++;
+      if (o.to_q) {
+        c_virt++;
+        continue;
+      }
+      if (o.to_x) {
+        c_cross++;
+        if (!CrossEvent(W, R, p)) break;
+        continue;
+      }
+      if (!o.to_c) break;  // step limit reached (flagged)
+      c_coll++;
+      bool alive, to_t;
+      CollideEvent(W, R, p, nullptr, nullptr, nullptr, alive, to_t);
+      if (to_t) {
+        c_tsl++;
+        if (!TslPodEvent(W, R, p)) break;
+        continue;
+      }
+      if (!alive) break;
+    }
+  }
+  if (c_tail) {
+    atomicAdd(R.ev + kEvAdvance, c_adv);
+    atomicAdd(R.ev + kEvVirtual, c_virt);
+    atomicAdd(R.ev + kEvCross, c_cross);
+    atomicAdd(R.ev + kEvCollide, c_coll);
+    atomicAdd(R.ev + kEvTsl, c_tsl);
+    atomicAdd(R.ev + kEvTail, c_tail);
   }
 }
 

@@ -39,15 +39,22 @@ namespace mmc {
 constexpr int kHid = 256;           // hidden width of the fused kernel
 constexpr int kFragElems = 8 * 16 * 64 * 8;  // bf16 per 256x256 layer (65536)
 constexpr int kInferWaves = 4;
+constexpr int kStage = 8;  // weight staging depth (16-byte loads per thread)
+static_assert((kFragElems / 8) % (kStage * 64 * kInferWaves) == 0, "staging tiles the layer");
+
+// The fused kernel's small parameters, FP32, one block staged in LDS:
+//   [kW0, +1024)  layer-0 weights as float4 [h 2][kb 16][q 8]
+//   [kB0, +256)   layer-0 bias [h][kb][q]
+//   [kB1, +256)   layer-1 bias [h 2][t 8][r 16]
+//   [kB2, +256)   layer-2 bias
+//   [kW3, +512)   layer-3 weights [h 2][t 8][r 16][2]
+constexpr int kW0 = 0, kB0 = 1024, kB1 = 1280, kB2 = 1536, kW3 = 1792, kSmall = 2304;
 
 // Device view of the network for the fused kernel.
 struct MlpDev {
   const bf16x8* frag1;  // layer 1 A fragments [t 8][kb 16][lane 64]
   const bf16x8* frag2;  // layer 2
-  const float* p0;      // layer 0: [h 2][kb 16][q 8][5] = w0[f][0..3], b0[f]
-  const float* bias1;   // [t 8][r 16][h 2]
-  const float* bias2;
-  const float* p3;      // layer 3: [h 2][t 8][r 16][2]
+  const float* small;   // kSmall floats, layout above
   float b3[2];
   float in_shift[4], in_scale[4], out_shift[2], out_scale[2];
 };
@@ -64,6 +71,11 @@ __host__ __device__ __forceinline__ int FeatC(int t, int r, int h) {
 }
 
 __device__ __forceinline__ float Silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU for the BF16 inference path: hardware exp2 and reciprocal (about
+// 1 ulp FP32 each, far below the BF16 rounding that follows).
+__device__ __forceinline__ float SiluFast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
 
 void MlpFree(MlpState& m) {
   void* ps[] = {m.params, m.wbf16, m.adam_m, m.adam_v, m.grads, m.act, m.loss,
@@ -80,13 +92,20 @@ void MlpFree(MlpState& m) {
 __global__ void __launch_bounds__(64 * kInferWaves)
     KMlpFused(MlpDev m, const float* __restrict__ x, float* __restrict__ y,
               const uint32_t* n_ptr, uint32_t n_static) {
-  extern __shared__ bf16x8 lds[];  // kFragElems bf16 = 128 KiB
+  extern __shared__ bf16x8 lds[];  // kFragElems bf16 = 128 KiB, then kSmall floats
+  float* sp = reinterpret_cast<float*>(lds + kFragElems / 8);
   const uint32_t n = n_ptr ? *n_ptr : n_static;
   const int wave = threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   const int h = lane / 32;
   const int j = lane % 32;
   const uint32_t rows_per_pass = 32u * kInferWaves;
+  if (blockIdx.x * rows_per_pass >= n) return;  // no rows: stage nothing
+  for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
+    reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(m.small)[i];
+  // (the first layer staging below begins with a barrier)
+  const float4* w0 = reinterpret_cast<const float4*>(sp + kW0) + h * 128;
+  const float* b0 = sp + kB0 + h * 128;
   for (uint32_t base = blockIdx.x * rows_per_pass; base < n; base += gridDim.x * rows_per_pass) {
     const uint32_t row = base + wave * 32 + j;
     const bool valid = row < n;
@@ -97,13 +116,14 @@ __global__ void __launch_bounds__(64 * kInferWaves)
     }
     // layer 0 on the vector ALUs, straight into B-fragment order
     bf16x8 B[16];
+    __syncthreads();  // small parameters staged (first pass)
 #pragma unroll
     for (int kb = 0; kb < 16; kb++) {
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        const float* w = m.p0 + ((h * 16 + kb) * 8 + q) * 5;
-        const float z = w[4] + w[0] * xn[0] + w[1] * xn[1] + w[2] * xn[2] + w[3] * xn[3];
-        B[kb][q] = (__bf16)Silu(z);
+        const float4 w = w0[kb * 8 + q];
+        const float z = b0[kb * 8 + q] + w.x * xn[0] + w.y * xn[1] + w.z * xn[2] + w.w * xn[3];
+        B[kb][q] = (__bf16)SiluFast(z);
       }
     }
     f32x16 acc[8];
@@ -112,7 +132,14 @@ __global__ void __launch_bounds__(64 * kInferWaves)
     for (int L = 0; L < 2; L++) {
       const bf16x8* src = L == 0 ? m.frag1 : m.frag2;
       __syncthreads();
-      for (int i = threadIdx.x; i < kFragElems / 8; i += blockDim.x) lds[i] = src[i];
+      // kStage 16-byte loads in flight per thread, then the LDS stores
+      for (int i0 = threadIdx.x; i0 < kFragElems / 8; i0 += kStage * 64 * kInferWaves) {
+        bf16x8 v[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; u++) v[u] = src[i0 + u * 64 * kInferWaves];
+#pragma unroll
+        for (int u = 0; u < kStage; u++) lds[i0 + u * 64 * kInferWaves] = v[u];
+      }
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < 8; t++)
@@ -126,12 +153,12 @@ __global__ void __launch_bounds__(64 * kInferWaves)
           acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kb], acc[t], 0, 0, 0);
         }
       }
-      const float* bias = L == 0 ? m.bias1 : m.bias2;
+      const float* bias = sp + (L == 0 ? kB1 : kB2) + h * 128;
 #pragma unroll
       for (int t = 0; t < 8; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-          const float a = Silu(acc[t][r] + bias[(t * 16 + r) * 2 + h]);
+          const float a = SiluFast(acc[t][r] + bias[t * 16 + r]);
           acc[t][r] = a;
           if (L == 0) B[2 * t + r / 8][r % 8] = (__bf16)a;
         }
@@ -142,9 +169,9 @@ __global__ void __launch_bounds__(64 * kInferWaves)
     for (int t = 0; t < 8; t++)
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const float* w = m.p3 + ((h * 8 + t) * 16 + r) * 2;
-        o0 += w[0] * acc[t][r];
-        o1 += w[1] * acc[t][r];
+        const float2 w = reinterpret_cast<const float2*>(sp + kW3)[(h * 8 + t) * 16 + r];
+        o0 += w.x * acc[t][r];
+        o1 += w.y * acc[t][r];
       }
     o0 += __shfl_xor(o0, 32);
     o1 += __shfl_xor(o1, 32);
@@ -158,7 +185,7 @@ __global__ void __launch_bounds__(64 * kInferWaves)
 // Packs the FP32 master parameters into the fused kernel's layouts.
 __global__ void KMlpPack(const float* P, int w0_off, int b0_off, int w1_off, int b1_off,
                          int w2_off, int b2_off, int w3_off, __bf16* frag1, __bf16* frag2,
-                         float* p0, float* bias1, float* bias2, float* p3) {
+                         float* small) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kFragElems) {
     // i = ((t * 16 + kb) * 64 + lane) * 8 + q
@@ -168,22 +195,19 @@ __global__ void KMlpPack(const float* P, int w0_off, int b0_off, int w1_off, int
     frag1[i] = (__bf16)P[w1_off + row * kHid + col];
     frag2[i] = (__bf16)P[w2_off + row * kHid + col];
   }
-  if (i < 2 * 16 * 8) {
+  if (i < 2 * 16 * 8) {  // i = (h * 16 + kb) * 8 + q
     const int q = i % 8, kb = (i / 8) % 16, h = i / 128;
     const int f = FeatB(kb, h, q);
-    for (int k = 0; k < 4; k++) p0[i * 5 + k] = P[w0_off + f * 4 + k];
-    p0[i * 5 + 4] = P[b0_off + f];
+    for (int k = 0; k < 4; k++) small[kW0 + i * 4 + k] = P[w0_off + f * 4 + k];
+    small[kB0 + i] = P[b0_off + f];
   }
-  if (i < 8 * 16 * 2) {
-    const int h = i % 2, r = (i / 2) % 16, t = i / 32;
-    bias1[i] = P[b1_off + FeatC(t, r, h)];
-    bias2[i] = P[b2_off + FeatC(t, r, h)];
-  }
-  if (i < 2 * 8 * 16) {
+  if (i < 2 * 8 * 16) {  // i = (h * 8 + t) * 16 + r
     const int r = i % 16, t = (i / 16) % 8, h = i / 128;
     const int f = FeatC(t, r, h);
-    p3[i * 2 + 0] = P[w3_off + 0 * kHid + f];
-    p3[i * 2 + 1] = P[w3_off + 1 * kHid + f];
+    small[kB1 + i] = P[b1_off + f];
+    small[kB2 + i] = P[b2_off + f];
+    small[kW3 + i * 2 + 0] = P[w3_off + 0 * kHid + f];
+    small[kW3 + i * 2 + 1] = P[w3_off + 1 * kHid + f];
   }
 }
 
@@ -398,13 +422,15 @@ __device__ void PodMap(const DevWorld& W, const DTsl& t, double E, double T, dou
   y2 = PodMapAlphaFrac(W, t, b, E, T, u2);
 }
 // (y1, y2) -> (alpha, beta) in the trained parametrisation.
-__device__ __forceinline__ void OutputsToAlphaBeta(double y1, double y2, double E, double T,
+// Returns false when the network's energy transfer leaves the kinematic
+// support (E' = E + beta kT <= 0, or not a number): the caller then uses the
+// POD sampler for that collision.
+__device__ __forceinline__ bool OutputsToAlphaBeta(double y1, double y2, double E, double T,
                                                    double awr, double beta_cutoff,
                                                    double& alpha, double& beta) {
   const double kT = kBoltzmann * T;
   double b = y1 * (E + kT) / kT;
-  const double b_min = -E / kT;
-  if (b < b_min) b = b_min;
+  if (!(E + b * kT > 0)) return false;
   if (b > beta_cutoff) b = beta_cutoff;
   const double Ep = E + b * kT;
   const double sE = sqrt(E), sp = sqrt(Ep > 0 ? Ep : 0.0);
@@ -413,6 +439,7 @@ __device__ __forceinline__ void OutputsToAlphaBeta(double y1, double y2, double 
   const double f = y2 < 0 ? 0 : (y2 > 1 ? 1 : y2);
   alpha = lo + f * (hi - lo);
   beta = b;
+  return true;
 }
 
 __global__ void KPodMap(DevWorld W, int tsl, uint32_t n, const float* x, float* y) {
@@ -430,7 +457,7 @@ __global__ void KPodMap(DevWorld W, int tsl, uint32_t n, const float* x, float* 
 // ---------------------------------------------------------------------------
 // Tracker step with the surrogate.
 __device__ __forceinline__ bool InBox(const MlpState& m, double E, double T) {
-  return E <= m.e_max && T >= m.t_lo && T <= m.t_hi;
+  return E >= m.e_min && E <= m.e_max && T >= m.t_lo && T <= m.t_hi;
 }
 __global__ void __launch_bounds__(256) KTslGather(DevWorld W, RunState R, MlpState m,
                                                   int iter) {
@@ -465,16 +492,16 @@ __global__ void __launch_bounds__(256) KTslApply(DevWorld W, RunState R, MlpStat
       const DTsl t = W.tsl[R.B.tslI[p]];
       double a = 0, b = 0;
       uint32_t e = 0;
-      if (InBox(m, E, T)) {
-        OutputsToAlphaBeta(m.yout[size_t(q) * 2], m.yout[size_t(q) * 2 + 1], E, T, t.awr,
-                           t.beta_cutoff, a, b);
+      if (InBox(m, E, T) && OutputsToAlphaBeta(m.yout[size_t(q) * 2],
+                                               m.yout[size_t(q) * 2 + 1], E, T, t.awr,
+                                               t.beta_cutoff, a, b)) {
         const double E_p = E + b * kBoltzmann * T;
         const double mu = (E + E_p - a * t.awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
         const double phi = 2 * kPi * Canonical(g);
         dir = Rotate(dir, mu > 1 ? 1.0 : (mu < -1 ? -1.0 : mu), phi);
-        E = E_p > 0 ? E_p : 1e-300;
+        E = E_p;
       } else {
-        fallback = true;  // outside the trained box: the POD sampler
+        fallback = true;  // outside the trained box or the support: the POD sampler
         e = TslScatterPod(W, t, g, T, dir, E, a, b);
       }
       if (e) {
@@ -499,12 +526,8 @@ static MlpDev DevView(const MlpState& m) {
   MlpDev d;
   d.frag1 = reinterpret_cast<const bf16x8*>(m.packed);
   d.frag2 = reinterpret_cast<const bf16x8*>(m.packed) + kFragElems / 8;
-  const float* f = reinterpret_cast<const float*>(
+  d.small = reinterpret_cast<const float*>(
       reinterpret_cast<const char*>(m.packed) + 2 * kFragElems * sizeof(__bf16));
-  d.p0 = f;
-  d.bias1 = f + 2 * 16 * 8 * 5;
-  d.bias2 = d.bias1 + 8 * 16 * 2;
-  d.p3 = d.bias2 + 8 * 16 * 2;
   d.b3[0] = m.b3[0];
   d.b3[1] = m.b3[1];
   for (int k = 0; k < 4; k++) {
@@ -521,7 +544,8 @@ static MlpDev DevView(const MlpState& m) {
 static void LaunchFused(const MlpState& m, const float* x, float* y, const uint32_t* n_ptr,
                         uint32_t n_static, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(KMlpFused, dim3(grid), dim3(64 * kInferWaves),
-                     kFragElems * sizeof(__bf16), s, DevView(m), x, y, n_ptr, n_static);
+                     kFragElems * sizeof(__bf16) + kSmall * sizeof(float), s, DevView(m), x, y,
+                     n_ptr, n_static);
 }
 
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
@@ -707,14 +731,10 @@ static int Repack(mmc_ctx* ctx) {
   char* base = reinterpret_cast<char*>(m.packed);
   __bf16* f1 = reinterpret_cast<__bf16*>(base);
   __bf16* f2 = f1 + kFragElems;
-  float* f = reinterpret_cast<float*>(base + 2 * kFragElems * sizeof(__bf16));
-  float* p0 = f;
-  float* b1 = p0 + 2 * 16 * 8 * 5;
-  float* b2 = b1 + 8 * 16 * 2;
-  float* p3 = b2 + 8 * 16 * 2;
+  float* small = reinterpret_cast<float*>(base + 2 * kFragElems * sizeof(__bf16));
   hipLaunchKernelGGL(KMlpPack, dim3((total + 255) / 256), dim3(256), 0, MmcStream(ctx),
                      m.params, m.w_off[0], m.b_off[0], m.w_off[1], m.b_off[1], m.w_off[2],
-                     m.b_off[2], m.w_off[3], f1, f2, p0, b1, b2, p3);
+                     m.b_off[2], m.w_off[3], f1, f2, small);
   // layer-3 bias lives in the kernel arguments
   float b3[2];
   if (hipMemcpyAsync(b3, m.params + m.b_off[3], sizeof(b3), hipMemcpyDeviceToHost,
@@ -751,6 +771,7 @@ extern "C" int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* d, const f
   std::memcpy(m.in_scale, d->in_scale, sizeof(m.in_scale));
   std::memcpy(m.out_shift, d->out_shift, sizeof(m.out_shift));
   std::memcpy(m.out_scale, d->out_scale, sizeof(m.out_scale));
+  m.e_min = d->e_min;
   m.e_max = d->e_max;
   m.t_lo = d->t_lo;
   m.t_hi = d->t_hi;
@@ -758,7 +779,7 @@ extern "C" int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* d, const f
   if (hipMalloc(&m.params, pbytes) != hipSuccess || hipMalloc(&m.grads, pbytes) != hipSuccess ||
       hipMalloc(&m.adam_m, pbytes) != hipSuccess || hipMalloc(&m.adam_v, pbytes) != hipSuccess ||
       hipMalloc(&m.loss, sizeof(float)) != hipSuccess ||
-      hipMalloc(&m.packed, 2 * kFragElems * sizeof(__bf16) + 4096 * sizeof(float)) != hipSuccess)
+      hipMalloc(&m.packed, 2 * kFragElems * sizeof(__bf16) + kSmall * sizeof(float)) != hipSuccess)
     return MmcFail(ctx, MMC_ERR_HIP, "mlp allocation failed");
   hipMemsetAsync(m.adam_m, 0, pbytes, MmcStream(ctx));
   hipMemsetAsync(m.adam_v, 0, pbytes, MmcStream(ctx));
@@ -958,10 +979,9 @@ __global__ void KSurrogateFinish(DevWorld W, MlpState m, int tsl, uint32_t n, co
   if (i >= n) return;
   const DTsl t = W.tsl[tsl];
   double a, b;
-  if (InBox(m, E[i], T[i])) {
-    OutputsToAlphaBeta(yout[size_t(i) * 2], yout[size_t(i) * 2 + 1], E[i], T[i], t.awr,
-                       t.beta_cutoff, a, b);
-  } else {
+  if (!InBox(m, E[i], T[i]) ||
+      !OutputsToAlphaBeta(yout[size_t(i) * 2], yout[size_t(i) * 2 + 1], E[i], T[i], t.awr,
+                          t.beta_cutoff, a, b)) {
     uint32_t g = rng[i];
     uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
     if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);

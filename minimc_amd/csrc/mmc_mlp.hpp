@@ -17,7 +17,7 @@ struct MlpState {
   int w_off[kMlpMaxLayers] = {}, b_off[kMlpMaxLayers] = {};
   long n_params = 0;
   float in_shift[4] = {}, in_scale[4] = {}, out_shift[2] = {}, out_scale[2] = {};
-  double e_max = 0, t_lo = 0, t_hi = 0;
+  double e_min = 0, e_max = 0, t_lo = 0, t_hi = 0;
   float* params = nullptr;        // FP32 master parameters
   void* wbf16 = nullptr;          // unused
   void* packed = nullptr;         // inference layouts (BF16 fragments + FP32)

@@ -39,6 +39,7 @@ struct mmc_ctx {
   bool has_tsl = false;
   bool fixed_fission = false;  // fixed-source problem with fission children
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
+  uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   // bank
   uint32_t cap = 0;
   RunState R{};
@@ -544,6 +545,21 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     *out = ctx;  // keep the message available through mmc_last_error
     return rc;
   }
+  // The CE kernels stage the energy hash (up to 32 KiB) into LDS once per
+  // block: cap their grid at the blocks that are resident at once, so that
+  // each block amortises the staging over several grid-stride passes.
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) ==
+          hipSuccess && n_cu > 0) {
+    const size_t lds = sizeof(int) * (ctx->n_hash0 + 1);
+    int occ_a = 0, occ_c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance, kBlock, lds) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide, kBlock, lds) !=
+            hipSuccess)
+      occ_a = occ_c = 2;
+    ctx->lds_grid = uint32_t(std::max(1, std::min(occ_a, occ_c))) * uint32_t(n_cu);
+  }
   uint64_t cap = ctx->opt.max_in_flight ? ctx->opt.max_in_flight : (1ull << 20);
   cap = std::min<uint64_t>(std::max<uint64_t>(cap, 1024), 1ull << 28);
   const int rb = AllocBank(ctx, uint32_t(cap));
@@ -728,8 +744,9 @@ static void ProfCollect(mmc_ctx* ctx) {
 static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
   const size_t lds = ctx->W.energy_mode == 1 && ctx->n_hash0 > 0
                          ? sizeof(int) * (ctx->n_hash0 + 1) : 0;
+  const uint32_t sgrid = lds ? std::min(grid, ctx->lds_grid) : grid;
   ProfBegin(ctx, MMC_K_ADVANCE);
-  hipLaunchKernelGGL(KAdvance, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
+  hipLaunchKernelGGL(KAdvance, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
                      iter, qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_CROSS);
@@ -737,7 +754,7 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
                      qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_COLLIDE);
-  hipLaunchKernelGGL(KCollide, dim3(grid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
+  hipLaunchKernelGGL(KCollide, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
                      iter, qin);
   ProfEnd(ctx);
   if (ctx->has_tsl) {
@@ -792,7 +809,27 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   int iter = 0, qin = 0;
   std::vector<uint32_t> ring(kRing * kRingStride);
   uint32_t misc[16];
+  // history-based tail (KTail) once few particles are left, where no
+  // collision can start a new particle
+  const bool tail_ok = ctx->opt.tail_threshold >= 0 && (ctx->R.keig || !ctx->fixed_fission);
+  const uint32_t tail_at =
+      ctx->opt.tail_threshold > 0
+          ? uint32_t(ctx->opt.tail_threshold)
+          : std::max<uint32_t>(1024u, std::min<uint32_t>(ctx->cap / 32u, 32768u));
   for (;;) {
+    if (tail_ok && bound < tail_at && iter > 0) {
+      ProfBegin(ctx, MMC_K_TAIL);
+      hipLaunchKernelGGL(KTail, dim3(Blocks(bound)), dim3(kBlock), 0, ctx->stream, ctx->W,
+                         ctx->R, iter, qin);
+      ProfEnd(ctx);
+      HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ProfCollect(ctx);
+      const int rc = CheckMisc(ctx, misc);
+      if (rc) return rc;
+      break;
+    }
     const uint32_t grid = Blocks(bound);
     for (int k = 0; k < K; k++) {
       const int rc = LaunchIteration(ctx, iter, qin, grid);

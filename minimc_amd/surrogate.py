@@ -6,7 +6,9 @@ targets come from the POD tables themselves: `mmc_tsl_pod_map` evaluates the
 POD inverse CDFs of beta (level u1) and alpha (level u2) on the device.  The
 forward and backward passes and the Adam steps are the library's own FP32
 kernels; inference is the fused BF16-MFMA kernel.  Outside the trained box
-(E > e_max or T outside [t_lo, t_hi]) the tracker falls back to POD.
+(E outside [e_min, e_max] or T outside [t_lo, t_hi]) and where the network's
+energy transfer leaves the kinematic support (E' <= 0), the tracker falls back
+to POD.
 """
 import ctypes as C
 import os
@@ -26,7 +28,7 @@ def n_params(widths=WIDTHS):
 
 
 def make_desc(out_shift=(0.0, 0.5), out_scale=(1.0, 0.3), widths=WIDTHS,
-              e_max=E_HI, t_lo=T_LO, t_hi=T_HI):
+              e_min=E_LO, e_max=E_HI, t_lo=T_LO, t_hi=T_HI):
     d = abi.MlpDesc()
     d.n_layers = len(widths) - 1
     for i, w in enumerate(widths):
@@ -39,7 +41,7 @@ def make_desc(out_shift=(0.0, 0.5), out_scale=(1.0, 0.3), widths=WIDTHS,
     for k in range(2):
         d.out_shift[k] = out_shift[k]
         d.out_scale[k] = out_scale[k]
-    d.e_max, d.t_lo, d.t_hi = e_max, t_lo, t_hi
+    d.e_min, d.e_max, d.t_lo, d.t_hi = e_min, e_max, t_lo, t_hi
     return d
 
 

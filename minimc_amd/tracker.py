@@ -232,11 +232,11 @@ class Tracker:
     def set_profiling(self, on=True):
         self._check(lib().mmc_set_profiling(self.h, 1 if on else 0))
 
-    KERNELS = ("advance", "cross", "collide", "tsl", "mlp")
+    KERNELS = ("advance", "cross", "collide", "tsl", "mlp", "tail")
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        n = (C.c_uint64 * 5)()
+        ms = np.zeros(len(self.KERNELS))
+        n = (C.c_uint64 * len(self.KERNELS))()
         self._check(lib().mmc_get_kernel_times(self.h, abi.dptr(ms), n))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNELS)}
 
