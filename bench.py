@@ -225,15 +225,20 @@ def tsl_batch_rates(tracker, n=1 << 22, reps=5, seed=4):
 def run_c5(args):
     from minimc_amd import abi, configs, surrogate
     from minimc_amd.tracker import Tracker
+    from minimc_amd import autotune
     pr = configs.c5_multizone()
     n = pr.histories
+    # launch parameters from the autotuner (short untimed runs of this problem)
+    t0 = time.perf_counter()
+    opts, table = autotune.tune(pr, n=n, cycles=3)
+    t_tune = time.perf_counter() - t0
     # plain POD tracker
-    tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n)
+    tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n, **opts)
     pod = _keig_leg(pr, tp, args.warmup, args.steps)
     tp.close()
     # surrogate tracker: the network is trained once (outside every timed
     # region) and cached in minimc_amd/_build
-    ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n)
+    ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n, **opts)
     t0 = time.perf_counter()
     net_path = os.path.join(ROOT, "minimc_amd", "_build", "tsl_surrogate.npz")
     trained_here = not os.path.exists(net_path)
@@ -265,6 +270,8 @@ def run_c5(args):
         "histories_per_s_ratio": sur["histories_per_s"] / pod["histories_per_s"],
         "k_diff_sigma": float(dk),
         "surrogate": sur, "pod": pod, "tsl_batch": batch,
+        "autotune": {"best": opts, "seconds": t_tune,
+                     "ms_per_cycle": [[o, s * 1e3] for o, s in table]},
         "network": {"trained_here": trained_here, "seconds": t_net,
                     "path": os.path.relpath(net_path, ROOT)},
         "roofline": _roofline("c5", kt, counts),

@@ -731,38 +731,59 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
 // ---- tail: history-based completion of the last live particles ------------
 // When few particles are left, one event per kernel launch leaves the GPU
 // idle between short launches; this kernel instead runs every remaining
-// particle to the end of its history in one thread, with the same event
-// functions in the same order (so the same random numbers and results).
-// Thermal scatterings use the POD sampler here.  Used only where a
-// collision cannot start a new particle (k-eigenvalue, or no fission).
+// particle to the end of its history, with the same event functions in the
+// same order (so the same random numbers and results).  Each lane holds one
+// particle and its next event; on every step the wave runs the event type
+// that most of its lanes wait for, so a step costs one event's latency
+// instead of the sum over the types present in the wave.  A lane whose
+// history ends takes the next queued particle.  Thermal scatterings use the
+// POD sampler here.  Used only where a collision cannot start a new particle
+// (k-eigenvalue, or no fission).
 __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, int qin) {
+  enum { kAdv = 0, kCrs = 1, kCol = 2, kTsl = 3, kDone = 4 };
   const uint32_t n = *Ring(R, iter - 1, kQ);
-  unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0, c_tail = 0;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const uint32_t p = R.Q[qin][q];
-    c_tail++;
-    for (;;) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  int st = q < n ? kAdv : kDone;
+  uint32_t p = st == kAdv ? R.Q[qin][q] : 0u;
+  unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0;
+  unsigned long long c_tail = st == kAdv ? 1 : 0;
+  for (;;) {
+    int pick = -1, most = 0;
+#pragma unroll
+    for (int k = kAdv; k <= kTsl; k++) {
+      const int c = __popcll(__ballot(st == k));
+      if (c > most) {
+        most = c;
+        pick = k;
+      }
+    }
+    if (pick < 0) break;  // every lane of the wave is done
+    if (st != pick) continue;
+    if (st == kAdv) {
       const AdvanceOut o = AdvanceEvent(W, R, p, nullptr);
       c_adv++;
-      if (o.to_q) {
-        c_virt++;
-        continue;
-      }
-      if (o.to_x) {
-        c_cross++;
-        if (!CrossEvent(W, R, p)) break;
-        continue;
-      }
-      if (!o.to_c) break;  // step limit reached (flagged)
+      c_virt += o.to_q ? 1 : 0;
+      st = o.to_q ? kAdv : (o.to_x ? kCrs : (o.to_c ? kCol : kDone));
+    } else if (st == kCrs) {
+      c_cross++;
+      st = CrossEvent(W, R, p) ? kAdv : kDone;
+    } else if (st == kCol) {
       c_coll++;
       bool alive, to_t;
       CollideEvent(W, R, p, nullptr, nullptr, nullptr, alive, to_t);
-      if (to_t) {
-        c_tsl++;
-        if (!TslPodEvent(W, R, p)) break;
-        continue;
+      st = to_t ? kTsl : (alive ? kAdv : kDone);
+    } else {
+      c_tsl++;
+      st = TslPodEvent(W, R, p) ? kAdv : kDone;
+    }
+    if (st == kDone) {
+      q += stride;
+      if (q < n) {
+        p = R.Q[qin][q];
+        st = kAdv;
+        c_tail++;
       }
-      if (!alive) break;
     }
   }
   if (c_tail) {

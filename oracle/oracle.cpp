@@ -18,15 +18,20 @@
 //                  infinite-medium k_inf closed form, point-source absorber
 //                  leakage e^-1 (test/test_FixedSource.cpp:13-26).
 //   continuous     lin-lin closed form on synthetic grids; free-gas cross
-//                  section adjustment closed form; free-gas kernel energy
-//                  moments (mean outgoing energy, E -> 2kT equilibrium).
+//                  section adjustment closed form; free-gas kernel: the
+//                  stationary collision-energy law (mean 7/4 kT at A = 1, by
+//                  quadrature) and the stationary-target limit.
+//   fission        Watt spectrum: mean 3a/2 + a^2 b/4 and a KS test against
+//                  the integrated Watt CDF on a k-eigenvalue bank.
 //   TSL POD        rank-full reconstruction equals the tabulated inverse
 //                  CDFs; sampled alpha/beta marginals against the analytic
 //                  S(alpha,beta) by KS/W1; alpha bounds closed form.
 //   tallies        mean / std-dev formula (src/Scorable.cpp:51-70) on
 //                  hand-made score streams.
 //   k-eigenvalue   no reference behaviour (src/KEigenvalue.cpp:36-62 is a
-//                  stub): k_inf closed form in an infinite medium.
+//                  stub): k_inf closed form in an infinite medium; Shannon
+//                  entropy recomputed from the bank in the test.  The
+//                  systematic resampling is parity unpinned.
 #include "oracle.h"
 
 #include <algorithm>

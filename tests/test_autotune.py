@@ -1,0 +1,26 @@
+"""Autotuner host logic (no GPU): problem signatures and the search."""
+from minimc_amd import autotune, configs
+
+
+def test_signature_is_stable_and_specific():
+    a = autotune._signature(configs.c1_slab(), 0, 1000)
+    assert a == autotune._signature(configs.c1_slab(), 0, 1000)
+    assert a != autotune._signature(configs.c1_slab(width=10.0), 0, 1000)
+    assert a != autotune._signature(configs.c1_slab(), 1, 1000)
+    assert a != autotune._signature(configs.c1_slab(), 0, 2000)
+
+
+def test_search_picks_the_fastest(monkeypatch, tmp_path):
+    # synthetic cost: best tail 16384, best readback interval 16
+    def cost(problem, opts, n, cycles, tsl_mode, prepare):
+        t = opts["tail_threshold"]
+        return (abs(t - 16384) if t > 0 else 1e6) + abs(opts["graph_iters"] - 16)
+    monkeypatch.setattr(autotune, "_time_run", cost)
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "at.json"))
+    best, table = autotune.tune(configs.c1_slab(), n=1000)
+    assert best == {"tail_threshold": 16384, "graph_iters": 16}
+    assert len(table) == len(autotune.TAIL_CANDIDATES) + len(autotune.ITER_CANDIDATES) - 1
+    # the cached result is returned without timing again
+    monkeypatch.setattr(autotune, "_time_run", None)
+    best2, _ = autotune.tune(configs.c1_slab(), n=1000)
+    assert best2 == best

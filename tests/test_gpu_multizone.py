@@ -8,6 +8,7 @@ import pytest
 from minimc_amd import abi, configs, surrogate
 from oracle.oracle import Oracle
 from tests.gpu_util import require_gpu
+from tests.test_oracle_continuous import bank_entropy, watt_check
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +48,12 @@ def test_multizone_pod_tracker_matches_oracle():
     eo = np.mean([s["entropy"] for s in so[4:]])
     et = np.mean([s["entropy"] for s in st[4:]])
     assert abs(eo - et) < 0.05, (eo, et)
+    # fission sites: Watt spectrum; entropy of the last bank
+    src = t.get_source()
+    a, b = configs.C5_WATT
+    watt_check(np.array([x.E for x in src]), a, b)
+    if st[-1]["n_sites"] < pr.histories:
+        assert st[-1]["entropy"] == pytest.approx(bank_entropy(src, pr.entropy_mesh), abs=1e-9)
     co, ct = o.counters(), t.counters()
     assert ct["tsl_collisions"] > 0 and ct["tsl_fallbacks"] == 0
     assert abs(ct["tsl_collisions"] - co["tsl_collisions"]) <= 0.02 * co["tsl_collisions"]

@@ -10,6 +10,13 @@
   stationary law is the collision density phi(E) sigma_eff(E), with
   phi ~ E exp(-E/kT) (detailed balance) and sigma_eff(E) ~ F(A E / kT), the
   free-gas broadening above; for A = 1 its mean is 7/4 kT (quadrature).
+- Shannon entropy of the fission bank (k-eigenvalue, new): recomputed from
+  the banked sites as -sum p log2 p over the 8x8x8 mesh.
+- Watt fission spectrum (an additive extension: the reference's children
+  keep the parent's energy, ContinuousReaction.cpp:258-264): the banked
+  sites of a k-eigenvalue cycle follow p(E) ~ exp(-E/a) sinh(sqrt(b E)),
+  mean 3a/2 + a^2 b/4, checked by the mean and by a KS test against the
+  CDF integrated numerically.
 """
 import math
 
@@ -127,3 +134,54 @@ def test_free_gas_stationary_target_limit():
     want = (A * A + 1) / (A + 1) ** 2
     se = s.E.std() / math.sqrt(n)
     assert abs(s.E.mean() - want) < 4 * se + 1e-6
+
+
+def watt_cdf(E, a, b):
+    """CDF of p(E) ~ exp(-E/a) sinh(sqrt(b E)) by trapezoidal quadrature."""
+    x = np.linspace(0.0, 60.0 * a, 400001)
+    pdf = np.exp(-x / a) * np.sinh(np.sqrt(b * x))
+    c = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(x))])
+    return np.interp(E, x, c / c[-1])
+
+
+def watt_check(E, a, b):
+    E = np.unique(E)  # resampled sources repeat bank sites
+    n = len(E)
+    mean = 1.5 * a + 0.25 * a * a * b
+    sd = math.sqrt(1.5 * a * a + 0.5 * a ** 3 * b)
+    assert abs(E.mean() - mean) < 4 * sd / math.sqrt(n), (E.mean(), mean)
+    Es = np.sort(E)
+    F = watt_cdf(Es, a, b)
+    ks = max(np.max(np.arange(1, n + 1) / n - F), np.max(F - np.arange(n) / n))
+    assert ks < 1.63 / math.sqrt(n), ks  # 1 % level
+    return n
+
+
+def bank_entropy(sites, mesh):
+    """-sum p log2 p of the distinct banked sites (k < 1: the resampled
+    source repeats every bank site at least once) on the entropy mesh."""
+    n, lo, hi = (np.asarray(v, dtype=float) for v in mesh)
+    allpos = np.array([[s.x, s.y, s.z, s.E] for s in sites])
+    pos = np.unique(allpos, axis=0)[:, :3]  # copies repeat position and energy
+    idx = np.clip(((pos - lo) / (hi - lo) * n).astype(int), 0, n.astype(int) - 1)
+    flat = (idx[:, 0] * int(n[1]) + idx[:, 1]) * int(n[2]) + idx[:, 2]
+    p = np.bincount(flat) / len(flat)
+    p = p[p > 0]
+    return float(-(p * np.log2(p)).sum())
+
+
+def test_watt_fission_spectrum_and_entropy():
+    pr = configs.c5_multizone(histories=20000)
+    o = Oracle(pr)
+    o.set_entropy_mesh(*pr.entropy_mesh)
+    o.keig_init(pr.histories)
+    st = o.run_cycle(active=False, threads=8)
+    src = o.get_source()
+    assert st["n_sites"] < pr.histories  # k < 1: every bank site is in the source
+    assert st["entropy"] == pytest.approx(bank_entropy(src, pr.entropy_mesh), abs=1e-12)
+    E = np.array([s.E for s in src])
+    a, b = configs.C5_WATT
+    assert watt_check(E, a, b) > 5000
+    # a plausible mistake (a and b swapped) fails the same check
+    with pytest.raises(AssertionError):
+        watt_check(E, b, a)
