@@ -559,12 +559,40 @@ __device__ __forceinline__ double FreeGasAdjust(double awr, double E, double T) 
   return (1 + 1 / (2 * arg)) * erf(sqrt(arg)) + exp(-arg) / sqrt(kPi * arg);
 }
 
+// Lookups repeated within one event reuse their results.  An event keeps
+// the last grid search (grid, E) -> index and the last thermal-scattering
+// total (table, E, T) -> value; the grid index found by the previous event
+// of the same particle comes in as a hint and is used only after checking
+// that it is the upper bound of E (x[i-1] <= E < x[i]), so every value
+// equals a fresh lookup's.
+struct XsMemo {
+  int grid = -1, gi = 0;
+  double E = 0;
+  int hint_grid = -1, hint_gi = 0;
+  int tsl = -1;
+  double tE = 0, tT = 0, tval = 0;
+};
+
 // Grid index of E for a nuclide whose tables share one grid, or -1.
 __device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNuclide& n,
-                                                const int* lds_hash, double E) {
+                                                const int* lds_hash, double E,
+                                                XsMemo* memo = nullptr) {
   if (n.grid < 0) return -1;
+  if (memo && memo->grid == n.grid && memo->E == E) return memo->gi;
   const DGrid g = W.grids[n.grid];
-  return GridUpperBound(W, g, lds_hash ? lds_hash : W.ipool + g.hash_off, E);
+  int gi = -1;
+  if (memo && memo->hint_grid == n.grid) {
+    const double* x = W.pool + g.x_off;
+    const int i = memo->hint_gi;
+    if (i >= 0 && i <= g.n && (i == 0 || x[i - 1] <= E) && (i == g.n || x[i] > E)) gi = i;
+  }
+  if (gi < 0) gi = GridUpperBound(W, g, lds_hash ? lds_hash : W.ipool + g.hash_off, E);
+  if (memo) {
+    memo->grid = n.grid;
+    memo->E = E;
+    memo->gi = gi;
+  }
+  return gi;
 }
 __device__ __forceinline__ double TabAt(const DevWorld& W, int t, int gi, double E) {
   return gi >= 0 ? TableAtI(W, t, gi, E) : TableAt(W, t, E);
@@ -572,12 +600,22 @@ __device__ __forceinline__ double TabAt(const DevWorld& W, int t, int gi, double
 // ContinuousScatter::GetCrossSection / GetMajorant (ContinuousReaction.cpp:77-117)
 __device__ __forceinline__ double RxXs(const DevWorld& W, const DCeNuclide& n,
                                        const DCeReaction& r, int gi, double E, double T,
-                                       bool majorant) {
+                                       bool majorant, XsMemo* memo = nullptr) {
   if (r.kind != 1) return TabAt(W, r.xs, gi, E);
   if (r.tsl >= 0) {
     const DTsl& t = W.tsl[r.tsl];
-    if (E < t.cutoff_energy)
-      return majorant ? TableAt(W, t.majorant, E) : TslTotal(W, t, E, T);
+    if (E < t.cutoff_energy) {
+      if (majorant) return TableAt(W, t.majorant, E);
+      if (memo && memo->tsl == r.tsl && memo->tE == E && memo->tT == T) return memo->tval;
+      const double v = TslTotal(W, t, E, T);
+      if (memo) {
+        memo->tsl = r.tsl;
+        memo->tE = E;
+        memo->tT = T;
+        memo->tval = v;
+      }
+      return v;
+    }
   }
   if (EvalValid(r.temperature, T)) return TabAt(W, r.xs, gi, E);
   if (FreeGasValid(n.awr, E, T))
@@ -595,19 +633,20 @@ __device__ __forceinline__ bool ModifiesTotal(const DevWorld& W, const DCeNuclid
 }
 // Continuous::GetTotal / GetMajorant (Continuous.cpp:26-55)
 __device__ __forceinline__ double CeTotal(const DevWorld& W, const DCeNuclide& n, int gi,
-                                          double E, double T, bool majorant) {
+                                          double E, double T, bool majorant,
+                                          XsMemo* memo = nullptr) {
   if (n.has_total && !ModifiesTotal(W, n, E) && EvalValid(n.total_temperature, T))
     return TabAt(W, n.total, gi, E);
   double acc = 0;
   for (int k = 0; k < n.rx_count; k++)
-    acc = acc + RxXs(W, n, W.rx[n.rx_begin + k], gi, E, T, majorant);
+    acc = acc + RxXs(W, n, W.rx[n.rx_begin + k], gi, E, T, majorant, memo);
   return acc;
 }
 
 // Microscopic material total (or majorant): Material.cpp:41-61.
 __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, double E,
                                                 double T, bool majorant,
-                                                const int* lds_hash) {
+                                                const int* lds_hash, XsMemo* memo = nullptr) {
   const DMaterial m = W.mats[mat];
   double acc = 0;
   if (W.energy_mode == 0) {
@@ -621,13 +660,15 @@ __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, doub
   for (int k = 0; k < m.comp_count; k++) {
     const DComp c = W.comps[m.comp_begin + k];
     const DCeNuclide n = W.ce[c.nuclide];
-    const int gi = NuclideGridIndex(W, n, lds_hash, E);
-    acc = acc + c.afrac * CeTotal(W, n, gi, E, T, majorant);
+    const int gi = NuclideGridIndex(W, n, lds_hash, E, memo);
+    acc = acc + c.afrac * CeTotal(W, n, gi, E, T, majorant, memo);
   }
   return acc;
 }
 // nu Sigma_f (microscopic, afrac-weighted) for the k estimators.
-__device__ __forceinline__ double MaterialNuFission(const DevWorld& W, int mat, double E) {
+__device__ __forceinline__ double MaterialNuFission(const DevWorld& W, int mat, double E,
+                                                    const int* lds_hash = nullptr,
+                                                    XsMemo* memo = nullptr) {
   const DMaterial m = W.mats[mat];
   double acc = 0;
   for (int k = 0; k < m.comp_count; k++) {
@@ -638,10 +679,11 @@ __device__ __forceinline__ double MaterialNuFission(const DevWorld& W, int mat, 
     } else {
       const DCeNuclide n = W.ce[c.nuclide];
       if (!n.fissile) continue;
+      const int gi = NuclideGridIndex(W, n, lds_hash, E, memo);
       double v = 0;
       for (int q = 0; q < n.rx_count; q++) {
         const DCeReaction r = W.rx[n.rx_begin + q];
-        if (r.kind == 2) v += TableAt(W, r.nubar, E) * TableAt(W, r.xs, E);
+        if (r.kind == 2) v += TableAt(W, r.nubar, E) * TabAt(W, r.xs, gi, E);
       }
       acc += c.afrac * v;
     }

@@ -32,6 +32,8 @@ struct DevBank {
   double *k_col, *k_trk, *k_abs;
   double* tslT;   // temperature of a deferred thermal-scattering collision
   int32_t* tslI;  // its thermal-scattering table
+  int32_t* xs_grid;  // last energy-grid search of the particle: grid (-1: none)
+  int32_t* xs_gi;    // and upper-bound index (a hint, verified before use)
 };
 
 struct RunState {
@@ -186,6 +188,7 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   R.B.surf[i] = -1;
   R.B.hist[i] = hist;
   R.B.steps[i] = 0;
+  R.B.xs_grid[i] = -1;
   ok = cell >= 0 && W.cells[cell].material >= 0;  // born in void: leaks at once
   }
   uint32_t tot;
@@ -212,6 +215,7 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
   R.B.surf[i] = -1;
   R.B.hist[i] = i;
   R.B.steps[i] = 0;
+  R.B.xs_grid[i] = -1;
   R.B.k_col[i] = 0;
   R.B.k_trk[i] = 0;
   R.B.k_abs[i] = 0;
@@ -272,7 +276,10 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   const DField fld = W.fields[c.field];
   const double aden = W.mats[c.material].aden;
   const double T = delta ? (fld.kind == 0 ? fld.c : fld.upper) : FieldAt(fld, pos);
-  const double sigma = MaterialTotal(W, c.material, E, T, delta, hash);
+  XsMemo memo;
+  memo.hint_grid = R.B.xs_grid[p];
+  memo.hint_gi = R.B.xs_gi[p];
+  const double sigma = MaterialTotal(W, c.material, E, T, delta, hash, &memo);
   const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
   int s;
   const double d_surf = NearestSurface(W, cell, pos, dir, s);
@@ -285,14 +292,19 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   } else {
     bool real = true;
     if (delta) {
-      const double tot = MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash);
+      const double tot =
+          MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash, &memo);
       real = Canonical(g) < tot / sigma;  // pre-flight position (quirk Q6)
     }
     d = d_coll;
     o.to_c = real;
     o.to_q = !real;
   }
-  if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E);
+  if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E, hash, &memo);
+  if (memo.grid >= 0) {
+    R.B.xs_grid[p] = memo.grid;
+    R.B.xs_gi[p] = memo.gi;
+  }
   pos = Add(pos, Mul(dir, d));
   R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
   R.B.rng[p] = g;
@@ -410,6 +422,7 @@ __device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, 
   R.B.surf[slot] = -1;
   R.B.hist[slot] = hist;
   R.B.steps[slot] = 0;
+  R.B.xs_grid[slot] = -1;
   if (live_q) live_q[atomicAdd(live_cnt, 1u)] = slot;
 }
 
@@ -469,13 +482,14 @@ struct CollideOut {
 __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, V3 pos,
                               V3& dir, double& E, uint32_t& g, int cell, uint32_t* live_q,
                               uint32_t* live_cnt, const int* hash, double* k_abs,
-                              int* tsl_index) {
+                              int* tsl_index, XsMemo* memo = nullptr) {
   CollideOut out{MMC_EV_SCATTER, 0, false};
   const DCell c = W.cells[cell];
   const DMaterial m = W.mats[c.material];
   const double T = FieldAt(W.fields[c.field], pos);
   // SampleNuclide
-  const double threshold_n = MaterialTotal(W, c.material, E, T, false, hash) * Canonical(g);
+  const double threshold_n =
+      MaterialTotal(W, c.material, E, T, false, hash, memo) * Canonical(g);
   int nuc = W.comps[m.comp_begin + m.comp_count - 1].nuclide;
   {
     double acc = 0;
@@ -486,7 +500,7 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
         tn = W.mg_total[cp.nuclide * W.G + int(E) - 1];
       } else {
         const DCeNuclide n = W.ce[cp.nuclide];
-        tn = CeTotal(W, n, NuclideGridIndex(W, n, hash, E), E, T, false);
+        tn = CeTotal(W, n, NuclideGridIndex(W, n, hash, E, memo), E, T, false, memo);
       }
       acc += cp.afrac * tn;
       if (acc > threshold_n) {
@@ -552,18 +566,18 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
     return out;
   }
   const DCeNuclide n = W.ce[nuc];
-  const int gi = NuclideGridIndex(W, n, hash, E);
+  const int gi = NuclideGridIndex(W, n, hash, E, memo);
   for (int attempt = 0;; attempt++) {  // Continuous::Interact; tail recursion (Q4)
     if (attempt >= kMaxReactionResamples) {
       out.event = MMC_EV_CAPTURE;
       out.children = -1;  // flags kErrReaction to the caller
       return out;
     }
-    const double threshold = Canonical(g) * CeTotal(W, n, gi, E, T, false);
+    const double threshold = Canonical(g) * CeTotal(W, n, gi, E, T, false, memo);
     double acc = 0;
     for (int k = 0; k < n.rx_count; k++) {
       const DCeReaction r = W.rx[n.rx_begin + k];
-      acc += RxXs(W, n, r, gi, E, T, false);
+      acc += RxXs(W, n, r, gi, E, T, false, memo);
       if (!(acc > threshold)) continue;
       if (r.kind == 0) {
         out.event = MMC_EV_CAPTURE;
@@ -622,15 +636,18 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   const int cell = R.B.cell[p];
   const DCell c = W.cells[cell];
   const double T = FieldAt(W.fields[c.field], pos);
+  XsMemo memo;
+  memo.hint_grid = R.B.xs_grid[p];
+  memo.hint_gi = R.B.xs_gi[p];
   if (W.n_est_flux || R.keig) {
-    const double tot = MaterialTotal(W, c.material, E, T, false, hash);
+    const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo);
     if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
-    if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E) / tot;
+    if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E, hash, &memo) / tot;
   }
   double kabs = 0;
   int tsl = -1;
   const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, live_q, live_cnt, hash,
-                               R.keig ? &kabs : nullptr, &tsl);
+                               R.keig ? &kabs : nullptr, &tsl, &memo);
   if (o.children < 0) atomicOr(R.misc, kErrReaction);
   if (R.keig) {
     R.B.k_abs[p] += kabs;

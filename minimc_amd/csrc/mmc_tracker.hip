@@ -40,6 +40,7 @@ struct mmc_ctx {
   bool fixed_fission = false;  // fixed-source problem with fission children
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
+  uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
   // bank
   uint32_t cap = 0;
   RunState R{};
@@ -492,6 +493,8 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   B.k_abs = DevAlloc<double>(ctx, cap);
   B.tslT = DevAlloc<double>(ctx, cap);
   B.tslI = DevAlloc<int32_t>(ctx, cap);
+  B.xs_grid = DevAlloc<int32_t>(ctx, cap);
+  B.xs_gi = DevAlloc<int32_t>(ctx, cap);
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
@@ -516,7 +519,8 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->scan_tmp = DevAlloc<uint32_t>(ctx, ctx->scan_cap);
   ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
-  if (!ctx->kpart || !B.tslI) return Fail(ctx, MMC_ERR_HIP, "out of device memory");
+  if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi)
+    return Fail(ctx, MMC_ERR_HIP, "out of device memory");
   HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
   R.max_steps = uint32_t(ctx->opt.max_steps ? std::min<uint64_t>(ctx->opt.max_steps, 0xffffffffull)
                                             : 1000000ull);
@@ -559,6 +563,12 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
             hipSuccess)
       occ_a = occ_c = 2;
     ctx->lds_grid = uint32_t(std::max(1, std::min(occ_a, occ_c))) * uint32_t(n_cu);
+    // the tail kernel's lanes take the next particle when one ends: launch
+    // only the blocks that are resident at once
+    int occ_t = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail, kBlock, 0) != hipSuccess)
+      occ_t = 1;
+    ctx->tail_grid = uint32_t(std::max(1, occ_t)) * uint32_t(n_cu);
   }
   uint64_t cap = ctx->opt.max_in_flight ? ctx->opt.max_in_flight : (1ull << 20);
   cap = std::min<uint64_t>(std::max<uint64_t>(cap, 1024), 1ull << 28);
@@ -819,8 +829,8 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
       ProfBegin(ctx, MMC_K_TAIL);
-      hipLaunchKernelGGL(KTail, dim3(Blocks(bound)), dim3(kBlock), 0, ctx->stream, ctx->W,
-                         ctx->R, iter, qin);
+      hipLaunchKernelGGL(KTail, dim3(std::min(Blocks(bound), ctx->tail_grid)), dim3(kBlock), 0,
+                         ctx->stream, ctx->W, ctx->R, iter, qin);
       ProfEnd(ctx);
       HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
                             ctx->stream));
