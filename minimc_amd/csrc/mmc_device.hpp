@@ -365,15 +365,27 @@ __device__ __forceinline__ double TableAtI(const DevWorld& W, int t, int hi, dou
 __device__ __forceinline__ int UpperBoundArr(const double* x, int n, double key) {
   return UpperBoundRange(x, 0, n, key);
 }
-// BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256)
-__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const DTslPart& P,
-                                               int cdf_i, int x_i, double T) {
+// The temperature bracket of BetaPartition/AlphaPartition::Evaluate
+// (ThermalScattering.cpp:183-256): the same for every evaluation of one
+// partition at one temperature, so a sample computes it once.
+struct TBracket {
+  int t_lo, t_hi;
+};
+__device__ __forceinline__ TBracket PartTBracket(const DevWorld& W, const DTslPart& P,
+                                                 double T) {
   const double* Ts = W.pool + P.T_off;
   const int cand = UpperBoundArr(Ts, P.nT, T);
   const bool above = cand == P.nT;
   const int t_hi = above ? cand - 1 : cand;
   const bool below = t_hi == 0;
-  const int t_lo = below ? t_hi : t_hi - 1;
+  return {below ? t_hi : t_hi - 1, t_hi};
+}
+// BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256)
+__device__ __forceinline__ double PartEvaluateAt(const DevWorld& W, const DTslPart& P,
+                                                 int cdf_i, int x_i, double T,
+                                                 TBracket br) {
+  const double* Ts = W.pool + P.T_off;
+  const int t_hi = br.t_hi, t_lo = br.t_lo;
   const double* S = W.pool + P.S_off;
   const double* cdf = W.pool + P.cdf_off + cdf_i * P.rank;
   const double* xh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
@@ -386,6 +398,10 @@ __device__ __forceinline__ double PartEvaluate(const DevWorld& W, const DTslPart
   }
   const double T_hi = Ts[t_hi], T_lo = Ts[t_lo];
   return v_lo + (v_hi - v_lo) / (T_hi - T_lo) * (T - T_lo);
+}
+__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const DTslPart& P,
+                                               int cdf_i, int x_i, double T) {
+  return PartEvaluateAt(W, P, cdf_i, x_i, T, PartTBracket(W, P, T));
 }
 // ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
 __device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, double E,
@@ -435,14 +451,16 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uin
   const DTslPart P = W.parts[t.bp_begin + p_i];
   const int local = p_i == 0 ? e_s : e_s - ends[p_i - 1];
   const double* F = W.pool + P.F_off;
+  const TBracket br = PartTBracket(W, P, T);
   for (int k = 0; k < kBetaResampleLimit; k++) {
     const double u = Canonical(g);
     const int f_hi = UpperBoundArr(F, P.nF, u);
     const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
     const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-    const double b_lo = f_hi != 0 ? PartEvaluate(W, P, f_hi - 1, local, T)
+    const double b_lo = f_hi != 0 ? PartEvaluateAt(W, P, f_hi - 1, local, T, br)
                                   : -E_s / (kBoltzmann * T);
-    const double b_hi = f_hi != P.nF ? PartEvaluate(W, P, f_hi, local, T) : t.beta_cutoff;
+    const double b_hi =
+        f_hi != P.nF ? PartEvaluateAt(W, P, f_hi, local, T, br) : t.beta_cutoff;
     const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
     const double b_min = -E / (kBoltzmann * T);
     if (b_min <= b) {
@@ -453,28 +471,49 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uin
   return false;
 }
 // find_cdf of SampleAlpha (ThermalScattering.cpp:395-418): upper_bound over
-// the CDF axis with a POD evaluation inside the comparator.
-__device__ __forceinline__ double FindCdf(const DevWorld& W, const DTslPart& P, int local,
-                                          double T, double a, double alpha_cutoff) {
+// the CDF axis with a POD evaluation inside the comparator (the reconstructed
+// values need not ascend, so this stays a bisection), then linear
+// interpolation of F.  SampleAlpha needs it at alpha_min and alpha_max; the
+// two bisections are independent and run in lockstep here so that their
+// memory round trips overlap.  Each follows exactly the probe sequence of a
+// single std::upper_bound.
+__device__ __forceinline__ void FindCdf2(const DevWorld& W, const DTslPart& P, int local,
+                                         double T, TBracket br, double a0, double a1,
+                                         double alpha_cutoff, double& F0, double& F1) {
   const double* F = W.pool + P.F_off;
   const int nF = P.nF;
-  int first = 0, len = nF;
-  while (len > 0) {
-    const int half = len >> 1;
-    const int mid = first + half;
-    if (a < PartEvaluate(W, P, mid, local, T)) {
-      len = half;
-    } else {
-      first = mid + 1;
-      len = len - half - 1;
+  int first0 = 0, len0 = nF, first1 = 0, len1 = nF;
+  while (len0 > 0 || len1 > 0) {
+    const int half0 = len0 >> 1, half1 = len1 >> 1;
+    const int mid0 = first0 + half0, mid1 = first1 + half1;
+    const double v0 = len0 > 0 ? PartEvaluateAt(W, P, mid0, local, T, br) : 0;
+    const double v1 = len1 > 0 ? PartEvaluateAt(W, P, mid1, local, T, br) : 0;
+    if (len0 > 0) {
+      if (a0 < v0) {
+        len0 = half0;
+      } else {
+        first0 = mid0 + 1;
+        len0 = len0 - half0 - 1;
+      }
+    }
+    if (len1 > 0) {
+      if (a1 < v1) {
+        len1 = half1;
+      } else {
+        first1 = mid1 + 1;
+        len1 = len1 - half1 - 1;
+      }
     }
   }
-  const int hi = first;
-  const double F_lo = hi != 0 ? F[hi - 1] : 0;
-  const double F_hi = hi != nF ? F[hi] : 1;
-  const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
-  const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : alpha_cutoff;
-  return F_lo + (a - a_lo) * (F_hi - F_lo) / (a_hi - a_lo);
+  const int h0 = first0, h1 = first1;
+  const double a_lo0 = h0 != 0 ? PartEvaluateAt(W, P, h0 - 1, local, T, br) : 0;
+  const double a_hi0 = h0 != nF ? PartEvaluateAt(W, P, h0, local, T, br) : alpha_cutoff;
+  const double a_lo1 = h1 != 0 ? PartEvaluateAt(W, P, h1 - 1, local, T, br) : 0;
+  const double a_hi1 = h1 != nF ? PartEvaluateAt(W, P, h1, local, T, br) : alpha_cutoff;
+  const double Fl0 = h0 != 0 ? F[h0 - 1] : 0, Fh0 = h0 != nF ? F[h0] : 1;
+  const double Fl1 = h1 != 0 ? F[h1 - 1] : 0, Fh1 = h1 != nF ? F[h1] : 1;
+  F0 = Fl0 + (a0 - a_lo0) * (Fh0 - Fl0) / (a_hi0 - a_lo0);
+  F1 = Fl1 + (a1 - a_lo1) * (Fh1 - Fl1) / (a_hi1 - a_lo1);
 }
 // Partition of a concatenated grid index (beta_partition_E_ends /
 // alpha_partition_beta_ends) and the index local to it.
@@ -522,15 +561,16 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t
   const int local = p_i == 0 ? b_s_i : b_s_i - ends[p_i - 1];
   const double* F = W.pool + P.F_off;
   const int nF = P.nF;
-  const double F_min = FindCdf(W, P, local, T, a_min, t.alpha_cutoff);
-  const double F_max = FindCdf(W, P, local, T, a_max, t.alpha_cutoff);
+  const TBracket br = PartTBracket(W, P, T);
+  double F_min, F_max;
+  FindCdf2(W, P, local, T, br, a_min, a_max, t.alpha_cutoff, F_min, F_max);
   for (int k = 0; k < kAlphaResampleLimit; k++) {
     const double u = F_min + Canonical(g) * (F_max - F_min);
     const int hi = UpperBoundArr(F, nF, u);
     const double F_lo = hi != 0 ? F[hi - 1] : 0;
     const double F_hi = hi != nF ? F[hi] : 1;
-    const double a_lo = hi != 0 ? PartEvaluate(W, P, hi - 1, local, T) : 0;
-    const double a_hi = hi != nF ? PartEvaluate(W, P, hi, local, T) : t.alpha_cutoff;
+    const double a_lo = hi != 0 ? PartEvaluateAt(W, P, hi - 1, local, T, br) : 0;
+    const double a_hi = hi != nF ? PartEvaluateAt(W, P, hi, local, T, br) : t.alpha_cutoff;
     const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
     if (a_min < a && a < a_max) {
       const double b_root = sqrt(E + b * kBoltzmann * T);

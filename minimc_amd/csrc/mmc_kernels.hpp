@@ -666,7 +666,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
 }
 
 #ifdef MMC_TRACKER_KERNELS
-__global__ void __launch_bounds__(256) KCollide(DevWorld W, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256, 2) KCollide(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter, kC);
   if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
