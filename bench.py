@@ -1,7 +1,7 @@
 """Benchmark of the MI355X particle-history tracker.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c2] [--no-cpu]
-                [--train-epochs E]
+                [--train-epochs E] [--tail-threshold N --graph-iters K]
 
 One step is one pass of the hot path over one batch of synthetic input.
 
@@ -228,9 +228,14 @@ def run_c5(args):
     from minimc_amd import autotune
     pr = configs.c5_multizone()
     n = pr.histories
-    # launch parameters from the autotuner (short untimed runs of this problem)
+    # launch parameters from the autotuner (short untimed runs of this
+    # problem), unless given on the command line
     t0 = time.perf_counter()
-    opts, table = autotune.tune(pr, n=n, cycles=3)
+    if args.tail_threshold is not None and args.graph_iters is not None:
+        opts, table = {"tail_threshold": args.tail_threshold,
+                       "graph_iters": args.graph_iters}, []
+    else:
+        opts, table = autotune.tune(pr, n=n, cycles=3)
     t_tune = time.perf_counter() - t0
     # plain POD tracker
     tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n, **opts)
@@ -287,6 +292,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--tail-threshold", type=int, default=None,
+                    help="with --graph-iters: skip the autotuner")
+    ap.add_argument("--graph-iters", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; "

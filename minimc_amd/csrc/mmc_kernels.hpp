@@ -16,6 +16,7 @@ namespace mmc {
 constexpr int kRing = 16;        // iteration counter ring
 constexpr int kRingStride = 8;   // counters per iteration
 enum RingSlot : int { kQ = 0, kC = 1, kX = 2, kT = 3 };
+constexpr int kMiscFallback = 5;
 // event counters (u64)
 enum Ev : int {
   kEvAdvance = 0, kEvCollide, kEvCross, kEvVirtual, kEvTsl, kEvFallback,
@@ -29,7 +30,7 @@ struct DevBank {
   int32_t* surf;
   uint32_t* hist;
   uint32_t* steps;
-  double *k_col, *k_trk, *k_abs;
+  double *k_col, *k_trk, *k_abs;  // k-eigenvalue sums, indexed by history
   double* tslT;   // temperature of a deferred thermal-scattering collision
   int32_t* tslI;  // its thermal-scattering table
   int32_t* xs_grid;  // last energy-grid search of the particle: grid (-1: none)
@@ -41,7 +42,8 @@ struct RunState {
   uint32_t* Q[2];
   uint32_t *QC, *QX, *QT;
   uint32_t* ring;      // kRing * kRingStride
-  uint32_t* misc;      // [0] err bits, [1] records, [2] sites, [3] slots
+  uint32_t* misc;      // [0] err bits, [1] records, [2] sites, [3] slots,
+                       // [kMiscFallback] parked surrogate fallbacks
   unsigned long long* ev;
   uint32_t capacity;   // slots
   int keig;
@@ -300,7 +302,8 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     o.to_c = real;
     o.to_q = !real;
   }
-  if (R.keig) R.B.k_trk[p] += d * aden * MaterialNuFission(W, c.material, E, hash, &memo);
+  if (R.keig)
+    R.B.k_trk[R.B.hist[p]] += d * aden * MaterialNuFission(W, c.material, E, hash, &memo);
   if (memo.grid >= 0) {
     R.B.xs_grid[p] = memo.grid;
     R.B.xs_gi[p] = memo.gi;
@@ -642,7 +645,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   if (W.n_est_flux || R.keig) {
     const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo);
     if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
-    if (R.keig) R.B.k_col[p] += MaterialNuFission(W, c.material, E, hash, &memo) / tot;
+    if (R.keig) R.B.k_col[R.B.hist[p]] += MaterialNuFission(W, c.material, E, hash, &memo) / tot;
   }
   double kabs = 0;
   int tsl = -1;
@@ -650,7 +653,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
                                R.keig ? &kabs : nullptr, &tsl, &memo);
   if (o.children < 0) atomicOr(R.misc, kErrReaction);
   if (R.keig) {
-    R.B.k_abs[p] += kabs;
+    R.B.k_abs[R.B.hist[p]] += kabs;
     if (o.event == MMC_EV_FISSION) R.fis_cnt[R.B.hist[p]] = uint32_t(o.children);
   }
   if (o.tsl) {
@@ -743,6 +746,34 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
     if (alive) R.Q[qin ^ 1][iq] = p;
     WaveCount(R.ev + kEvTsl, valid);
   }
+}
+
+// ---- bank compaction ----------------------------------------------------------
+// As histories end, the live particles spread over ever more of the bank, and
+// a wave's loads of one field touch one cache line per lane.  This kernel
+// moves the live queue's particles to slots 0..n-1 of the other bank (the
+// host then swaps the banks), so that the following iterations read the
+// particle state with unit stride.  The k-eigenvalue sums are indexed by
+// history and stay where they are; tslT/tslI are only live within one
+// iteration.
+__global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin) {
+  const uint32_t n = *Ring(R, iter - 1, kQ);
+  uint32_t* Q = R.Q[qin];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = Q[i];
+    dst.x[i] = R.B.x[p]; dst.y[i] = R.B.y[p]; dst.z[i] = R.B.z[p];
+    dst.u[i] = R.B.u[p]; dst.v[i] = R.B.v[p]; dst.w[i] = R.B.w[p];
+    dst.E[i] = R.B.E[p];
+    dst.rng[i] = R.B.rng[p];
+    dst.cell[i] = R.B.cell[p];
+    dst.surf[i] = R.B.surf[p];
+    dst.hist[i] = R.B.hist[p];
+    dst.steps[i] = R.B.steps[p];
+    dst.xs_grid[i] = R.B.xs_grid[p];
+    dst.xs_gi[i] = R.B.xs_gi[p];
+    Q[i] = i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n;  // slots in use
 }
 
 // ---- tail: history-based completion of the last live particles ------------

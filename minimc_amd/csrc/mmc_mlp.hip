@@ -38,9 +38,7 @@ namespace mmc {
 
 constexpr int kHid = 256;           // hidden width of the fused kernel
 constexpr int kFragElems = 8 * 16 * 64 * 8;  // bf16 per 256x256 layer (65536)
-constexpr int kInferWaves = 4;
 constexpr int kStage = 8;  // weight staging depth (16-byte loads per thread)
-static_assert((kFragElems / 8) % (kStage * 64 * kInferWaves) == 0, "staging tiles the layer");
 
 // The fused kernel's small parameters, FP32, one block staged in LDS:
 //   [kW0, +1024)  layer-0 weights as float4 [h 2][kb 16][q 8]
@@ -79,44 +77,142 @@ __device__ __forceinline__ float SiluFast(float z) {
 
 void MlpFree(MlpState& m) {
   void* ps[] = {m.params, m.wbf16, m.adam_m, m.adam_v, m.grads, m.act, m.loss,
-                m.xin, m.yout, m.fb, m.packed};
+                m.xin, m.yout, m.fb, m.packed, m.h1};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   m = MlpState{};
 }
 
 // ---------------------------------------------------------------------------
-// Fused BF16 MFMA inference.  Block = kInferWaves waves, each wave owns 32
-// rows per pass; the loop is block-uniform (barriers inside).  `n_ptr`, when
-// set, points at a device row count (the tracker's TSL queue length).
-__global__ void __launch_bounds__(64 * kInferWaves)
-    KMlpFused(MlpDev m, const float* __restrict__ x, float* __restrict__ y,
-              const uint32_t* n_ptr, uint32_t n_static) {
+// BF16 MFMA inference in two persistent kernels, one per 256 x 256 layer, so
+// that a block stages its layer's weights in LDS once per launch:
+//   KMlpA: inputs -> layer 0 (VALU) -> layer 1 (MFMA) -> H1, the layer-1
+//          activations in BF16, stored in layer 2's B-fragment order
+//          ([32-row tile][kb 16][lane 64] x 8 bf16)
+//   KMlpB: H1 -> layer 2 (MFMA) -> layer 3 (VALU) -> outputs
+// Each wave owns whole 32-row tiles and runs without block barriers after
+// the staging; a block holds kMlpWaves waves (two per SIMD, so that one
+// wave's matrix instructions overlap another's vector work).  In the tracker
+// KMlpA gathers (ln E, T, u1, u2) of the thermal-scattering queue and KMlpB
+// applies the scatter (TrackIO below).  The rare rows the network cannot
+// serve wait in a fallback queue that the POD sampler empties once per batch
+// of iterations, so that its latency is not paid every iteration.
+constexpr int kMlpWaves = 8;
+constexpr uint32_t kTileBf16x8 = 16 * 64;  // bf16x8 per 32-row tile of H1
+
+// Stages one layer's A fragments and the small parameters (block-wide).
+__device__ __forceinline__ void StageLayer(bf16x8* lds, float* sp, const bf16x8* src,
+                                           const float* small) {
+  for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
+    reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(small)[i];
+  for (int i0 = threadIdx.x; i0 < kFragElems / 8; i0 += kStage * blockDim.x) {
+    bf16x8 v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) v[u] = src[i0 + u * blockDim.x];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) lds[i0 + u * blockDim.x] = v[u];
+  }
+  __syncthreads();
+}
+// acc = W_layer (from LDS) x B over the 256 input features.
+__device__ __forceinline__ void MfmaLayer(const bf16x8* lds, int lane, const bf16x8 B[16],
+                                          f32x16 acc[8]) {
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
+#pragma unroll
+  for (int kb = 0; kb < 16; kb++) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const bf16x8 a = lds[(t * 16 + kb) * 64 + lane];
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kb], acc[t], 0, 0, 0);
+    }
+  }
+}
+
+// Returns false when the network's energy transfer leaves the kinematic
+// support (E' = E + beta kT <= 0, or not a number): the caller then uses the
+// POD sampler for that collision.
+__device__ __forceinline__ bool OutputsToAlphaBeta(double y1, double y2, double E, double T,
+                                                   double awr, double beta_cutoff,
+                                                   double& alpha, double& beta) {
+  const double kT = kBoltzmann * T;
+  double b = y1 * (E + kT) / kT;
+  if (!(E + b * kT > 0)) return false;
+  if (b > beta_cutoff) b = beta_cutoff;
+  const double Ep = E + b * kT;
+  const double sE = sqrt(E), sp = sqrt(Ep > 0 ? Ep : 0.0);
+  const double akT = awr * kT;
+  const double lo = (sE - sp) * (sE - sp) / akT, hi = (sE + sp) * (sE + sp) / akT;
+  const double f = y2 < 0 ? 0 : (y2 > 1 ? 1 : y2);
+  alpha = lo + f * (hi - lo);
+  beta = b;
+  return true;
+}
+
+// Tracker-mode arguments of the two kernels.
+struct TrackIO {
+  DevWorld W;
+  RunState R;
+  int iter, qin;
+  uint32_t* fb;  // fallback queue, length in RunState::misc[kMiscFallback]
+  double e_min, e_max, t_lo, t_hi;
+};
+__device__ __forceinline__ bool InBoxT(const TrackIO& io, double E, double T) {
+  return E >= io.e_min && E <= io.e_max && T >= io.t_lo && T <= io.t_hi;
+}
+
+template <bool kTrack>
+__global__ void __launch_bounds__(64 * kMlpWaves)
+    KMlpA(MlpDev m, const float* __restrict__ x, const uint32_t* n_ptr, uint32_t n_static,
+          bf16x8* __restrict__ h1, TrackIO io) {
   extern __shared__ bf16x8 lds[];  // kFragElems bf16 = 128 KiB, then kSmall floats
   float* sp = reinterpret_cast<float*>(lds + kFragElems / 8);
   const uint32_t n = n_ptr ? *n_ptr : n_static;
+  const uint32_t n_tiles = (n + 31) / 32;
+  if (blockIdx.x * kMlpWaves >= n_tiles) return;  // no rows: stage nothing
+  StageLayer(lds, sp, m.frag1, m.small);
   const int wave = threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   const int h = lane / 32;
   const int j = lane % 32;
-  const uint32_t rows_per_pass = 32u * kInferWaves;
-  if (blockIdx.x * rows_per_pass >= n) return;  // no rows: stage nothing
-  for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
-    reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(m.small)[i];
-  // (the first layer staging below begins with a barrier)
   const float4* w0 = reinterpret_cast<const float4*>(sp + kW0) + h * 128;
   const float* b0 = sp + kB0 + h * 128;
-  for (uint32_t base = blockIdx.x * rows_per_pass; base < n; base += gridDim.x * rows_per_pass) {
-    const uint32_t row = base + wave * 32 + j;
-    const bool valid = row < n;
-    float xn[4];
-    for (int k = 0; k < 4; k++) {
-      const float v = valid ? x[size_t(row) * 4 + k] : 0.0f;
-      xn[k] = (v - m.in_shift[k]) * m.in_scale[k];
+  const float* bias = sp + kB1 + h * 128;
+  for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < n_tiles;
+       tile += gridDim.x * kMlpWaves) {
+    // compiler-only memory barrier: the weights in LDS are re-read every
+    // tile instead of being hoisted into (and spilled from) registers
+    __asm__ volatile("" ::: "memory");
+    const uint32_t row = tile * 32 + j;
+    float xr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (row < n) {
+      if (kTrack) {
+        // KTslGather of the event loop: (ln E, T, u1, u2) of queued particle p
+        const uint32_t p = io.R.QT[row];
+        const double E = io.R.B.E[p];
+        const double T = io.R.B.tslT[p];
+        uint32_t g = io.R.B.rng[p];
+        const double u1 = Canonical(g), u2 = Canonical(g);
+        if (h == 0) io.R.B.rng[p] = g;
+        xr[0] = float(log(E));
+        xr[1] = float(T);
+        xr[2] = float(u1);
+        xr[3] = float(u2);
+      } else {
+        const float4 v = reinterpret_cast<const float4*>(x)[row];
+        xr[0] = v.x;
+        xr[1] = v.y;
+        xr[2] = v.z;
+        xr[3] = v.w;
+      }
     }
+    float xn[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) xn[k] = (xr[k] - m.in_shift[k]) * m.in_scale[k];
     // layer 0 on the vector ALUs, straight into B-fragment order
     bf16x8 B[16];
-    __syncthreads();  // small parameters staged (first pass)
 #pragma unroll
     for (int kb = 0; kb < 16; kb++) {
 #pragma unroll
@@ -127,58 +223,106 @@ __global__ void __launch_bounds__(64 * kInferWaves)
       }
     }
     f32x16 acc[8];
-    // two 256 x 256 layers on the matrix cores
+    MfmaLayer(lds, lane, B, acc);
+    // bias + SiLU, rounded to BF16 in layer 2's B-fragment order
 #pragma unroll
-    for (int L = 0; L < 2; L++) {
-      const bf16x8* src = L == 0 ? m.frag1 : m.frag2;
-      __syncthreads();
-      // kStage 16-byte loads in flight per thread, then the LDS stores
-      for (int i0 = threadIdx.x; i0 < kFragElems / 8; i0 += kStage * 64 * kInferWaves) {
-        bf16x8 v[kStage];
+    for (int t = 0; t < 8; t++)
 #pragma unroll
-        for (int u = 0; u < kStage; u++) v[u] = src[i0 + u * 64 * kInferWaves];
+      for (int r = 0; r < 16; r++)
+        B[2 * t + r / 8][r % 8] = (__bf16)SiluFast(acc[t][r] + bias[t * 16 + r]);
+    bf16x8* out = h1 + size_t(tile) * kTileBf16x8 + lane;
 #pragma unroll
-        for (int u = 0; u < kStage; u++) lds[i0 + u * 64 * kInferWaves] = v[u];
-      }
-      __syncthreads();
+    for (int kb = 0; kb < 16; kb++) out[kb * 64] = B[kb];
+  }
+}
+
+template <bool kTrack>
+__global__ void __launch_bounds__(64 * kMlpWaves)
+    KMlpB(MlpDev m, float* __restrict__ y, const uint32_t* n_ptr, uint32_t n_static,
+          const bf16x8* __restrict__ h1, TrackIO io) {
+  extern __shared__ bf16x8 lds[];
+  float* sp = reinterpret_cast<float*>(lds + kFragElems / 8);
+  const uint32_t n = n_ptr ? *n_ptr : n_static;
+  const uint32_t n_tiles = (n + 31) / 32;
+  if (blockIdx.x * kMlpWaves >= n_tiles) return;
+  StageLayer(lds, sp, m.frag2, m.small);
+  const int wave = threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  const int h = lane / 32;
+  const int j = lane % 32;
+  const float* bias = sp + kB2 + h * 128;
+  const float2* w3 = reinterpret_cast<const float2*>(sp + kW3) + h * 128;
+  for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < n_tiles;
+       tile += gridDim.x * kMlpWaves) {
+    // compiler-only memory barrier: the weights in LDS are re-read every
+    // tile instead of being hoisted into (and spilled from) registers
+    __asm__ volatile("" ::: "memory");
+    const uint32_t row = tile * 32 + j;
+    const bool valid = row < n;
+    bf16x8 B[16];
+    const bf16x8* in = h1 + size_t(tile) * kTileBf16x8 + lane;
 #pragma unroll
-      for (int t = 0; t < 8; t++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < 16; kb++) {
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-          const bf16x8 a = lds[(t * 16 + kb) * 64 + lane];
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kb], acc[t], 0, 0, 0);
-        }
-      }
-      const float* bias = sp + (L == 0 ? kB1 : kB2) + h * 128;
-#pragma unroll
-      for (int t = 0; t < 8; t++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-          const float a = SiluFast(acc[t][r] + bias[t * 16 + r]);
-          acc[t][r] = a;
-          if (L == 0) B[2 * t + r / 8][r % 8] = (__bf16)a;
-        }
-    }
-    // layer 3 (256 -> 2) from the FP32 activations, lane halves combined
+    for (int kb = 0; kb < 16; kb++) B[kb] = in[kb * 64];
+    f32x16 acc[8];
+    MfmaLayer(lds, lane, B, acc);
+    // bias + SiLU, then layer 3 (256 -> 2) in FP32, lane halves combined
     float o0 = 0.0f, o1 = 0.0f;
 #pragma unroll
     for (int t = 0; t < 8; t++)
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const float2 w = reinterpret_cast<const float2*>(sp + kW3)[(h * 8 + t) * 16 + r];
-        o0 += w.x * acc[t][r];
-        o1 += w.y * acc[t][r];
+        const float a = SiluFast(acc[t][r] + bias[t * 16 + r]);
+        const float2 w = w3[t * 16 + r];
+        o0 += w.x * a;
+        o1 += w.y * a;
       }
     o0 += __shfl_xor(o0, 32);
     o1 += __shfl_xor(o1, 32);
-    if (valid && h == 0) {
-      y[size_t(row) * 2 + 0] = (o0 + m.b3[0]) * m.out_scale[0] + m.out_shift[0];
-      y[size_t(row) * 2 + 1] = (o1 + m.b3[1]) * m.out_scale[1] + m.out_shift[1];
+    const float y1 = (o0 + m.b3[0]) * m.out_scale[0] + m.out_shift[0];
+    const float y2 = (o1 + m.b3[1]) * m.out_scale[1] + m.out_shift[1];
+    if (!kTrack) {
+      if (valid && h == 0) {
+        y[size_t(row) * 2 + 0] = y1;
+        y[size_t(row) * 2 + 1] = y2;
+      }
+      continue;
     }
+    // KTslApply of the event loop: the scatter with the network's (alpha,
+    // beta); a particle outside the trained box or the kinematic support is
+    // parked in the fallback queue for the POD sampler (MlpFlushFallback)
+    const RunState& R = io.R;
+    bool alive = false, fall = false;
+    uint32_t p = 0;
+    if (valid && h == 0) {
+      p = R.QT[row];
+      double E = R.B.E[p];
+      const double T = R.B.tslT[p];
+      const DTsl t = io.W.tsl[R.B.tslI[p]];
+      double a = 0, b = 0;
+      if (InBoxT(io, E, T) &&
+          OutputsToAlphaBeta(y1, y2, E, T, t.awr, t.beta_cutoff, a, b)) {
+        V3 dir = LoadDir(R.B, p);
+        uint32_t g = R.B.rng[p];
+        const double E_p = E + b * kBoltzmann * T;
+        const double mu = (E + E_p - a * t.awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
+        const double phi = 2 * kPi * Canonical(g);
+        dir = Rotate(dir, mu > 1 ? 1.0 : (mu < -1 ? -1.0 : mu), phi);
+        if (io.W.n_est_tsl) ScoreAlphaBeta(io.W, R, R.B.hist[p], a, b);
+        R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+        R.B.E[p] = E_p;
+        R.B.rng[p] = g;
+        alive = true;
+      } else {
+        fall = true;
+      }
+    }
+    uint32_t tot;
+    const uint32_t iq = WavePush(Ring(R, io.iter, kQ), alive, tot);
+    if (alive) R.Q[io.qin ^ 1][iq] = p;
+    const uint32_t ifb = WavePush(R.misc + kMiscFallback, fall, tot);
+    if (fall) io.fb[ifb] = p;
+    WaveCount(R.ev + kEvTsl, valid && h == 0);
+    WaveCount(R.ev + kEvFallback, fall);
   }
 }
 
@@ -422,26 +566,6 @@ __device__ void PodMap(const DevWorld& W, const DTsl& t, double E, double T, dou
   y2 = PodMapAlphaFrac(W, t, b, E, T, u2);
 }
 // (y1, y2) -> (alpha, beta) in the trained parametrisation.
-// Returns false when the network's energy transfer leaves the kinematic
-// support (E' = E + beta kT <= 0, or not a number): the caller then uses the
-// POD sampler for that collision.
-__device__ __forceinline__ bool OutputsToAlphaBeta(double y1, double y2, double E, double T,
-                                                   double awr, double beta_cutoff,
-                                                   double& alpha, double& beta) {
-  const double kT = kBoltzmann * T;
-  double b = y1 * (E + kT) / kT;
-  if (!(E + b * kT > 0)) return false;
-  if (b > beta_cutoff) b = beta_cutoff;
-  const double Ep = E + b * kT;
-  const double sE = sqrt(E), sp = sqrt(Ep > 0 ? Ep : 0.0);
-  const double akT = awr * kT;
-  const double lo = (sE - sp) * (sE - sp) / akT, hi = (sE + sp) * (sE + sp) / akT;
-  const double f = y2 < 0 ? 0 : (y2 > 1 ? 1 : y2);
-  alpha = lo + f * (hi - lo);
-  beta = b;
-  return true;
-}
-
 __global__ void KPodMap(DevWorld W, int tsl, uint32_t n, const float* x, float* y) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -459,68 +583,25 @@ __global__ void KPodMap(DevWorld W, int tsl, uint32_t n, const float* x, float* 
 __device__ __forceinline__ bool InBox(const MlpState& m, double E, double T) {
   return E >= m.e_min && E <= m.e_max && T >= m.t_lo && T <= m.t_hi;
 }
-__global__ void __launch_bounds__(256) KTslGather(DevWorld W, RunState R, MlpState m,
-                                                  int iter) {
-  const uint32_t n = *Ring(R, iter, kT);
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const uint32_t p = R.QT[q];
-    const double E = R.B.E[p];
-    const double T = R.B.tslT[p];
-    uint32_t g = R.B.rng[p];
-    const double u1 = Canonical(g), u2 = Canonical(g);
-    R.B.rng[p] = g;
-    m.xin[size_t(q) * 4 + 0] = float(log(E));
-    m.xin[size_t(q) * 4 + 1] = float(T);
-    m.xin[size_t(q) * 4 + 2] = float(u1);
-    m.xin[size_t(q) * 4 + 3] = float(u2);
-  }
-}
-__global__ void __launch_bounds__(256) KTslApply(DevWorld W, RunState R, MlpState m, int iter,
-                                                 int qin) {
-  const uint32_t n = *Ring(R, iter, kT);
+// The POD sampler for the parked fallback particles; the survivors join the
+// live queue of iteration `iter` (Q[qin], length in ring slot iter - 1).
+__global__ void __launch_bounds__(256) KTslPodFallback(DevWorld W, RunState R, int iter,
+                                                        int qin, const uint32_t* fb) {
+  const uint32_t n = R.misc[kMiscFallback];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
-    const bool valid = q < n;
-    bool alive = false, fallback = false;
+    bool alive = false;
     uint32_t p = 0;
-    if (valid) {
-      p = R.QT[q];
-      V3 dir = LoadDir(R.B, p);
-      double E = R.B.E[p];
-      const double T = R.B.tslT[p];
-      uint32_t g = R.B.rng[p];
-      const DTsl t = W.tsl[R.B.tslI[p]];
-      double a = 0, b = 0;
-      uint32_t e = 0;
-      if (InBox(m, E, T) && OutputsToAlphaBeta(m.yout[size_t(q) * 2],
-                                               m.yout[size_t(q) * 2 + 1], E, T, t.awr,
-                                               t.beta_cutoff, a, b)) {
-        const double E_p = E + b * kBoltzmann * T;
-        const double mu = (E + E_p - a * t.awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
-        const double phi = 2 * kPi * Canonical(g);
-        dir = Rotate(dir, mu > 1 ? 1.0 : (mu < -1 ? -1.0 : mu), phi);
-        E = E_p;
-      } else {
-        fallback = true;  // outside the trained box or the support: the POD sampler
-        e = TslScatterPod(W, t, g, T, dir, E, a, b);
-      }
-      if (e) {
-        atomicOr(R.misc, e);
-      } else {
-        alive = true;
-        if (W.n_est_tsl) ScoreAlphaBeta(W, R, R.B.hist[p], a, b);
-        R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
-        R.B.E[p] = E;
-        R.B.rng[p] = g;
-      }
+    if (q < n) {
+      p = fb[q];
+      alive = TslPodEvent(W, R, p);
     }
     uint32_t tot;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
-    if (alive) R.Q[qin ^ 1][iq] = p;
-    WaveCount(R.ev + kEvTsl, valid);
-    WaveCount(R.ev + kEvFallback, fallback);
+    const uint32_t iq = WavePush(Ring(R, iter - 1, kQ), alive, tot);
+    if (alive) R.Q[qin][iq] = p;
   }
 }
+__global__ void KClearFallback(RunState R) { R.misc[kMiscFallback] = 0; }
 
 static MlpDev DevView(const MlpState& m) {
   MlpDev d;
@@ -541,24 +622,63 @@ static MlpDev DevView(const MlpState& m) {
   return d;
 }
 
-static void LaunchFused(const MlpState& m, const float* x, float* y, const uint32_t* n_ptr,
-                        uint32_t n_static, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(KMlpFused, dim3(grid), dim3(64 * kInferWaves),
-                     kFragElems * sizeof(__bf16) + kSmall * sizeof(float), s, DevView(m), x, y,
-                     n_ptr, n_static);
+static constexpr size_t kMlpLds = kFragElems * sizeof(__bf16) + kSmall * sizeof(float);
+static uint32_t MlpGrid(uint64_t rows) {
+  const uint64_t blocks = ((rows + 31) / 32 + kMlpWaves - 1) / kMlpWaves;
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(blocks, 256)));  // <= 1 per CU
+}
+// Batch inference of n rows (h1 holds at least n rows).
+static void LaunchInfer(const MlpState& m, const float* x, float* y, uint32_t n, bf16x8* h1,
+                        hipStream_t s) {
+  const TrackIO none{};
+  hipLaunchKernelGGL(KMlpA<false>, dim3(MlpGrid(n)), dim3(64 * kMlpWaves), kMlpLds, s,
+                     DevView(m), x, nullptr, n, h1, none);
+  hipLaunchKernelGGL(KMlpB<false>, dim3(MlpGrid(n)), dim3(64 * kMlpWaves), kMlpLds, s,
+                     DevView(m), y, nullptr, n, h1, none);
+}
+// H1 rows available for batch inference (grown on demand, at most 2^20 rows
+// per chunk).
+static int EnsureH1(mmc_ctx* ctx, MlpState& m, uint64_t rows) {
+  rows = std::min<uint64_t>(rows, 1u << 20);
+  if (m.h1_rows >= rows) return MMC_OK;
+  if (m.h1) (void)hipFree(m.h1);
+  m.h1 = nullptr;
+  m.h1_rows = 0;
+  const uint64_t tiles = (rows + 31) / 32;
+  if (hipMalloc(&m.h1, tiles * kTileBf16x8 * sizeof(bf16x8)) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "surrogate activation buffer allocation failed");
+  m.h1_rows = uint32_t(tiles * 32);
+  return MMC_OK;
+}
+static int InferChunked(mmc_ctx* ctx, MlpState& m, const float* x, float* y, uint64_t n) {
+  const int rc = EnsureH1(ctx, m, n);
+  if (rc) return rc;
+  for (uint64_t c0 = 0; c0 < n; c0 += m.h1_rows) {
+    const uint32_t c = uint32_t(std::min<uint64_t>(n - c0, m.h1_rows));
+    LaunchInfer(m, x + c0 * 4, y + c0 * 2, c, static_cast<bf16x8*>(m.h1), MmcStream(ctx));
+  }
+  return MMC_OK;
 }
 
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
                uint32_t grid, hipStream_t s, void (*mark)(void*, int), void* mark_arg) {
-  if (!m.ready || !m.xin) return 1;
-  mark(mark_arg, MMC_K_TSL);
-  hipLaunchKernelGGL(KTslGather, dim3(grid), dim3(256), 0, s, W, R, m, iter);
-  // the fused kernel reads the queue length itself; 256 blocks = 1 per CU
+  if (!m.ready || !m.h1 || !m.fb) return 1;
+  TrackIO io{W, R, iter, qin, m.fb, m.e_min, m.e_max, m.t_lo, m.t_hi};
+  // both kernels read the queue length themselves: one block per CU
   mark(mark_arg, MMC_K_MLP);
-  LaunchFused(m, m.xin, m.yout, Ring(R, iter, kT), 0, 256, s);
-  mark(mark_arg, MMC_K_TSL);
-  hipLaunchKernelGGL(KTslApply, dim3(grid), dim3(256), 0, s, W, R, m, iter, qin);
+  hipLaunchKernelGGL(KMlpA<true>, dim3(256), dim3(64 * kMlpWaves), kMlpLds, s, DevView(m),
+                     nullptr, Ring(R, iter, kT), 0u, static_cast<bf16x8*>(m.h1), io);
+  hipLaunchKernelGGL(KMlpB<true>, dim3(256), dim3(64 * kMlpWaves), kMlpLds, s, DevView(m),
+                     nullptr, Ring(R, iter, kT), 0u, static_cast<const bf16x8*>(m.h1), io);
   mark(mark_arg, -1);
+  return 0;
+}
+
+int MlpFlushFallback(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
+                     hipStream_t s) {
+  if (!m.ready || !m.fb) return 1;
+  hipLaunchKernelGGL(KTslPodFallback, dim3(64), dim3(256), 0, s, W, R, iter, qin, m.fb);
+  hipLaunchKernelGGL(KClearFallback, dim3(1), dim3(1), 0, s, R);
   return 0;
 }
 
@@ -827,9 +947,8 @@ extern "C" int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y)
   if (!m.ready) return MmcFail(ctx, MMC_ERR_UNSUPPORTED, "no 4-256-256-256-2 network set");
   if (n == 0) return MMC_OK;
   if (n > 0xffffffffull) return MmcFail(ctx, MMC_ERR_ARG, "batch too large");
-  const uint32_t rows_per_block = 32u * kInferWaves;
-  const uint32_t grid = uint32_t(std::min<uint64_t>((n + rows_per_block - 1) / rows_per_block, 1024));
-  LaunchFused(m, x, y, nullptr, uint32_t(n), grid, MmcStream(ctx));
+  const int rc = InferChunked(ctx, m, x, y, n);
+  if (rc) return rc;
   return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
 }
 
@@ -1014,9 +1133,12 @@ extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const
   const uint32_t nb = uint32_t((n + 255) / 256);
   hipLaunchKernelGGL(KSurrogateBatch, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
                      uint32_t(n), E, T, rng, xin);
-  const uint32_t rows_per_block = 32u * kInferWaves;
-  LaunchFused(m, xin, yout, nullptr, uint32_t(n),
-              uint32_t(std::min<uint64_t>((n + rows_per_block - 1) / rows_per_block, 1024)), s);
+  if (InferChunked(ctx, m, xin, yout, n)) {
+    (void)hipFree(xin);
+    (void)hipFree(yout);
+    (void)hipFree(err);
+    return MMC_ERR_HIP;
+  }
   hipLaunchKernelGGL(KSurrogateFinish, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
                      uint32_t(n), E, T, rng, yout, alpha, beta, err);
   uint32_t herr = 0;
@@ -1034,12 +1156,14 @@ extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const
 int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap) {
   MlpState& m = MmcMlp(ctx);
   if (m.stage_cap >= cap) return MMC_OK;
-  if (m.xin) (void)hipFree(m.xin);
-  if (m.yout) (void)hipFree(m.yout);
-  m.xin = nullptr;
-  m.yout = nullptr;
-  if (hipMalloc(&m.xin, sizeof(float) * 4 * size_t(cap)) != hipSuccess ||
-      hipMalloc(&m.yout, sizeof(float) * 2 * size_t(cap)) != hipSuccess)
+  if (m.h1) (void)hipFree(m.h1);
+  if (m.fb) (void)hipFree(m.fb);
+  m.h1 = nullptr;
+  m.fb = nullptr;
+  m.h1_rows = 0;
+  const uint64_t tiles = (uint64_t(cap) + 31) / 32;
+  if (hipMalloc(&m.h1, tiles * kTileBf16x8 * sizeof(bf16x8)) != hipSuccess ||
+      hipMalloc(&m.fb, sizeof(uint32_t) * size_t(cap)) != hipSuccess)
     return MmcFail(ctx, MMC_ERR_HIP, "surrogate staging allocation failed");
   m.stage_cap = cap;
   return MMC_OK;

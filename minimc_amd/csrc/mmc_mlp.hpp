@@ -27,10 +27,12 @@ struct MlpState {
   size_t act_cap = 0;
   float* loss = nullptr;
   long adam_t = 0;
-  float* xin = nullptr;           // tracker staging [cap x 4]
-  float* yout = nullptr;          // [cap x 2]
-  uint32_t* fb = nullptr;         // fallback queue
-  uint32_t stage_cap = 0;
+  float* xin = nullptr;           // unused
+  float* yout = nullptr;          // unused
+  uint32_t* fb = nullptr;         // parked fallback particles [cap]
+  uint32_t stage_cap = 0;         // rows of the tracker buffers
+  void* h1 = nullptr;             // layer-1 activations (BF16 B fragments)
+  uint32_t h1_rows = 0;
 };
 
 void MlpFree(MlpState& m);
@@ -39,12 +41,17 @@ struct mmc_ctx;
 // Staging buffers of the surrogate step for `cap` particle slots.
 int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap);
 namespace mmc {
-// One thermal-scattering step of the tracker with the surrogate: gathers
-// (log E, T, u1, u2) of the TSL queue, runs the network, applies the
-// scatter, and sends out-of-box inputs to the POD sampler.
+// One thermal-scattering step of the tracker with the surrogate: two network
+// kernels; the first gathers (log E, T, u1, u2) of the TSL queue, the second
+// applies the scatter (the POD sampler serves the rows the network cannot).
 // `mark(k)` is called before each launch with the profiling kind of the
 // next kernel (MMC_K_TSL or MMC_K_MLP) and with -1 after the last one.
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
                uint32_t grid, hipStream_t stream, void (*mark)(void*, int), void* mark_arg);
+// The POD sampler for the particles the surrogate parked; they rejoin the
+// live queue of iteration `iter` (Q[qin]).  Called before every read of the
+// live count, so that a parked particle is never missed.
+int MlpFlushFallback(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
+                     hipStream_t stream);
 
 }  // namespace mmc

@@ -41,6 +41,7 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
+  DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
   RunState R{};
@@ -495,6 +496,24 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   B.tslI = DevAlloc<int32_t>(ctx, cap);
   B.xs_grid = DevAlloc<int32_t>(ctx, cap);
   B.xs_gi = DevAlloc<int32_t>(ctx, cap);
+  // the compaction target: the moved fields get their own arrays, the
+  // history-indexed and transient ones are shared
+  DevBank& A = ctx->alt;
+  A = B;
+  A.x = DevAlloc<double>(ctx, cap);
+  A.y = DevAlloc<double>(ctx, cap);
+  A.z = DevAlloc<double>(ctx, cap);
+  A.u = DevAlloc<double>(ctx, cap);
+  A.v = DevAlloc<double>(ctx, cap);
+  A.w = DevAlloc<double>(ctx, cap);
+  A.E = DevAlloc<double>(ctx, cap);
+  A.rng = DevAlloc<uint32_t>(ctx, cap);
+  A.cell = DevAlloc<int32_t>(ctx, cap);
+  A.surf = DevAlloc<int32_t>(ctx, cap);
+  A.hist = DevAlloc<uint32_t>(ctx, cap);
+  A.steps = DevAlloc<uint32_t>(ctx, cap);
+  A.xs_grid = DevAlloc<int32_t>(ctx, cap);
+  A.xs_gi = DevAlloc<int32_t>(ctx, cap);
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
@@ -519,7 +538,9 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->scan_tmp = DevAlloc<uint32_t>(ctx, ctx->scan_cap);
   ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
-  if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi)
+  if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x || !A.y || !A.z || !A.u ||
+      !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.surf || !A.hist || !A.steps ||
+      !A.xs_grid || !A.xs_gi)
     return Fail(ctx, MMC_ERR_HIP, "out of device memory");
   HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
   R.max_steps = uint32_t(ctx->opt.max_steps ? std::min<uint64_t>(ctx->opt.max_steps, 0xffffffffull)
@@ -804,12 +825,29 @@ extern "C" int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* la
   return MMC_OK;
 }
 
+// Exchanges the particle fields that KCompact moves between the two banks.
+static void SwapBanks(mmc_ctx* ctx) {
+  DevBank& B = ctx->R.B;
+  DevBank& A = ctx->alt;
+  std::swap(B.x, A.x); std::swap(B.y, A.y); std::swap(B.z, A.z);
+  std::swap(B.u, A.u); std::swap(B.v, A.v); std::swap(B.w, A.w);
+  std::swap(B.E, A.E);
+  std::swap(B.rng, A.rng);
+  std::swap(B.cell, A.cell);
+  std::swap(B.surf, A.surf);
+  std::swap(B.hist, A.hist);
+  std::swap(B.steps, A.steps);
+  std::swap(B.xs_grid, A.xs_grid);
+  std::swap(B.xs_gi, A.xs_gi);
+}
+
 // Runs event iterations until no particle is alive.  The live-queue length
 // is read back every `K` iterations; in between, kernels read queue lengths
 // from device memory and grid-stride over them.
 static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   const int K = ctx->opt.graph_iters > 0 ? ctx->opt.graph_iters : 8;
-  if (ctx->has_tsl && ctx->opt.tsl_mode == MMC_TSL_SURROGATE) {
+  const bool surrogate = ctx->has_tsl && ctx->opt.tsl_mode == MMC_TSL_SURROGATE;
+  if (surrogate) {
     if (!ctx->mlp.ready)
       return Fail(ctx, MMC_ERR_UNSUPPORTED, "surrogate mode without a network (mmc_mlp_set)");
     const int rc = MlpEnsureStaging(ctx, ctx->cap);
@@ -826,6 +864,10 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       ctx->opt.tail_threshold > 0
           ? uint32_t(ctx->opt.tail_threshold)
           : std::max<uint32_t>(1024u, std::min<uint32_t>(ctx->cap / 32u, 32768u));
+  // compaction (KCompact) whenever the live particles fill less than half of
+  // the slots in use; only where no collision allocates new slots
+  const bool compact_ok = ctx->R.keig || !ctx->fixed_fission;
+  uint32_t in_use = n0;
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
       ProfBegin(ctx, MMC_K_TAIL);
@@ -847,6 +889,12 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       iter++;
       qin ^= 1;
     }
+    if (surrogate) {  // parked fallback particles rejoin before the count is read
+      ProfBegin(ctx, MMC_K_TSL);
+      const int rc = MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->stream);
+      ProfEnd(ctx);
+      if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
+    }
     HIPCHK(hipMemcpyAsync(ring.data(), ctx->R.ring, sizeof(uint32_t) * ring.size(),
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
@@ -859,6 +907,12 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     const uint32_t live = ring[((iter - 1 + kRing) % kRing) * kRingStride + kQ];
     if (live == 0) break;
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
+    if (compact_ok && 2 * live < in_use) {
+      hipLaunchKernelGGL(KCompact, dim3(Blocks(live)), dim3(kBlock), 0, ctx->stream, ctx->R,
+                         ctx->alt, iter, qin);
+      SwapBanks(ctx);
+      in_use = live;
+    }
   }
   // the next run starts with ring slot -1 = kRing-1
   if (iter % kRing != 0) {
