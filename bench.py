@@ -106,6 +106,19 @@ def cpu_baseline(cfg, threads, target_s=12.0):
     raise ValueError(cfg)
 
 
+def _profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of `workload` (profiles/<workload>_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_hbm_traffic.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def _roofline(cfg, kt, counts):
     """Roofline of the event kernel with the largest total time (the
     history-based tail kernel is latency-bound by construction and is
@@ -120,8 +133,12 @@ def _roofline(cfg, kt, counts):
                 "flop_per_sample": MLP_FLOP_PER_SAMPLE,
                 "avg_launch_ms": ms / max(launches, 1)}
     achieved = counts[dom] * BYTES[cfg][dom] / (ms * 1e-3) / 1e9
+    traffic = _profiled_traffic(dom, "c5_multizone") if cfg == "c5" else None
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE, "
+                            "profiles/c5_multizone_hbm_traffic.md)" if traffic else None,
+            "algorithmic_bytes_per_launch": counts[dom] * BYTES[cfg][dom] / max(launches, 1),
             "bytes_per_event": BYTES[cfg][dom], "avg_launch_ms": ms / max(launches, 1)}
 
 
