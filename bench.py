@@ -250,7 +250,8 @@ def run_c5(args):
     t0 = time.perf_counter()
     if args.tail_threshold is not None and args.graph_iters is not None:
         opts, table = {"tail_threshold": args.tail_threshold,
-                       "graph_iters": args.graph_iters}, []
+                       "graph_iters": args.graph_iters,
+                "compact_interval": args.compact_interval}, []
     else:
         opts, table = autotune.tune(pr, n=n, cycles=3)
     t_tune = time.perf_counter() - t0
@@ -312,6 +313,7 @@ def main():
     ap.add_argument("--tail-threshold", type=int, default=None,
                     help="with --graph-iters: skip the autotuner")
     ap.add_argument("--graph-iters", type=int, default=None)
+    ap.add_argument("--compact-interval", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; "

@@ -273,7 +273,10 @@ typedef struct mmc_options {
                              -1 = never)                                */
   uint64_t max_steps;   /* per-history event limit (0 = 1e6)            */
   int32_t quirks;       /* mmc_quirk bits (0 = corrected behaviour)      */
-  int32_t pad;
+  int32_t compact_interval; /* bank compaction: 0 = when the live particles
+                             fill less than half of the slots in use (at a
+                             read of the queue lengths), k > 0 = before
+                             every k-th iteration, -1 = never             */
 } mmc_options;
 /* Reference behaviours that are reproduced only on request.
  * MMC_QUIRK_NUDGE_CROSSING: after a surface crossing the new cell is the

@@ -152,7 +152,7 @@ class Options(C.Structure):
                 ("device", C.c_int32), ("block", C.c_int32),
                 ("max_in_flight", C.c_uint64), ("graph_iters", C.c_int32),
                 ("tail_threshold", C.c_int32), ("max_steps", C.c_uint64),
-                ("quirks", C.c_int32), ("pad", C.c_int32)]
+                ("quirks", C.c_int32), ("compact_interval", C.c_int32)]
 
 
 class ParticleSoA(C.Structure):

@@ -7,9 +7,12 @@ problem (history length distribution, table sizes) and on the device:
                   kernel finishes the batch (-1: never)
   graph_iters     event iterations launched between two host reads of the
                   live-queue length
+  compact_interval  bank compaction policy (0: when half empty, k: every k
+                  iterations, -1: never)
 
 `tune()` times short runs of the actual problem for each candidate
-(coordinate descent: the tail threshold first, then the readback interval),
+(coordinate descent: the tail threshold, the readback interval, then the
+compaction policy),
 and returns the fastest option set.  Results are cached per problem in
 minimc_amd/_build/autotune.json (that folder is built per machine).
 
@@ -24,6 +27,7 @@ from . import abi
 
 TAIL_CANDIDATES = (-1, 4096, 16384, 32768, 65536, 131072, 262144)
 ITER_CANDIDATES = (4, 8, 16, 32)
+COMPACT_CANDIDATES = (-1, 1, 2, 4)  # 0 (compact when half empty) is the start
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "autotune.json")
 
 
@@ -77,7 +81,7 @@ def _time_run(problem, opts, n, cycles, tsl_mode, prepare):
 
 def tune(problem, n=None, cycles=3, tsl_mode=abi.MMC_TSL_POD, prepare=None,
          use_cache=True, log=None):
-    """Returns ({"tail_threshold": .., "graph_iters": ..}, table of timings)."""
+    """Returns ({"tail_threshold": .., "graph_iters": .., ...}, table of timings)."""
     n = n or problem.histories
     key = _signature(problem, tsl_mode, n)
     cache = {}
@@ -99,11 +103,20 @@ def tune(problem, n=None, cycles=3, tsl_mode=abi.MMC_TSL_POD, prepare=None,
     for it in ITER_CANDIDATES:
         if it == best["graph_iters"]:
             continue
-        opts = {"tail_threshold": best["tail_threshold"], "graph_iters": it}
+        opts = dict(best, graph_iters=it)
         s = _time_run(problem, opts, n, cycles, tsl_mode, prepare)
         table.append((dict(opts), s))
         if log:
             log("tail %6d iters %2d: %.2f ms" % (opts["tail_threshold"], it, s * 1e3))
+        if s < best_s:
+            best, best_s = dict(opts), s
+    for cp in COMPACT_CANDIDATES:
+        opts = dict(best, compact_interval=cp)
+        s = _time_run(problem, opts, n, cycles, tsl_mode, prepare)
+        table.append((dict(opts), s))
+        if log:
+            log("tail %6d iters %2d compact %2d: %.2f ms" % (
+                opts["tail_threshold"], opts["graph_iters"], cp, s * 1e3))
         if s < best_s:
             best, best_s = dict(opts), s
     if use_cache:

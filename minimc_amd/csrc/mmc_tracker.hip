@@ -864,9 +864,11 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       ctx->opt.tail_threshold > 0
           ? uint32_t(ctx->opt.tail_threshold)
           : std::max<uint32_t>(1024u, std::min<uint32_t>(ctx->cap / 32u, 32768u));
-  // compaction (KCompact) whenever the live particles fill less than half of
-  // the slots in use; only where no collision allocates new slots
-  const bool compact_ok = ctx->R.keig || !ctx->fixed_fission;
+  // compaction (KCompact): by default whenever the live particles fill less
+  // than half of the slots in use, else every compact_interval iterations;
+  // only where no collision allocates new slots
+  const int every = ctx->opt.compact_interval;
+  const bool compact_ok = every >= 0 && (ctx->R.keig || !ctx->fixed_fission);
   uint32_t in_use = n0;
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
@@ -884,6 +886,11 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     }
     const uint32_t grid = Blocks(bound);
     for (int k = 0; k < K; k++) {
+      if (compact_ok && every > 0 && iter > 0 && iter % every == 0) {
+        hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
+                           ctx->alt, iter, qin);
+        SwapBanks(ctx);
+      }
       const int rc = LaunchIteration(ctx, iter, qin, grid);
       if (rc) return rc;
       iter++;
@@ -907,7 +914,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     const uint32_t live = ring[((iter - 1 + kRing) % kRing) * kRingStride + kQ];
     if (live == 0) break;
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
-    if (compact_ok && 2 * live < in_use) {
+    if (compact_ok && every == 0 && 2 * live < in_use) {
       hipLaunchKernelGGL(KCompact, dim3(Blocks(live)), dim3(kBlock), 0, ctx->stream, ctx->R,
                          ctx->alt, iter, qin);
       SwapBanks(ctx);

@@ -11,15 +11,19 @@ def test_signature_is_stable_and_specific():
 
 
 def test_search_picks_the_fastest(monkeypatch, tmp_path):
-    # synthetic cost: best tail 16384, best readback interval 16
+    # synthetic cost: best tail 16384, best readback interval 16, compaction
+    # every 2 iterations
     def cost(problem, opts, n, cycles, tsl_mode, prepare):
         t = opts["tail_threshold"]
-        return (abs(t - 16384) if t > 0 else 1e6) + abs(opts["graph_iters"] - 16)
+        c = opts.get("compact_interval", 0)
+        return ((abs(t - 16384) if t > 0 else 1e6) + abs(opts["graph_iters"] - 16)
+                + (0 if c == 2 else 0.5))
     monkeypatch.setattr(autotune, "_time_run", cost)
     monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "at.json"))
     best, table = autotune.tune(configs.c1_slab(), n=1000)
-    assert best == {"tail_threshold": 16384, "graph_iters": 16}
-    assert len(table) == len(autotune.TAIL_CANDIDATES) + len(autotune.ITER_CANDIDATES) - 1
+    assert best == {"tail_threshold": 16384, "graph_iters": 16, "compact_interval": 2}
+    assert len(table) == (len(autotune.TAIL_CANDIDATES) + len(autotune.ITER_CANDIDATES) - 1
+                          + len(autotune.COMPACT_CANDIDATES))
     # the cached result is returned without timing again
     monkeypatch.setattr(autotune, "_time_run", None)
     best2, _ = autotune.tune(configs.c1_slab(), n=1000)
