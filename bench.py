@@ -259,14 +259,13 @@ def run_c5(args):
     tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n, **opts)
     pod = _keig_leg(pr, tp, args.warmup, args.steps)
     tp.close()
-    # surrogate tracker: the network is trained once (outside every timed
-    # region) and cached in minimc_amd/_build
+    # surrogate tracker: the shipped network (minimc_amd/data), or one trained
+    # here outside every timed region when that is missing
     ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n, **opts)
     t0 = time.perf_counter()
-    net_path = os.path.join(ROOT, "minimc_amd", "_build", "tsl_surrogate.npz")
-    trained_here = not os.path.exists(net_path)
-    surrogate.cached_network(ts, path=net_path, epochs=args.train_epochs, batch=16384,
-                             lr=3e-3)
+    net_path = surrogate.SHIPPED
+    _, trained_here = surrogate.cached_network(ts, path=net_path, epochs=args.train_epochs,
+                                               batch=32768, lr=3e-3)
     t_net = time.perf_counter() - t0
     sur = _keig_leg(pr, ts, args.warmup, args.steps)
     batch = tsl_batch_rates(ts)

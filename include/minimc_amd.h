@@ -491,6 +491,8 @@ typedef struct mmc_train_opts {
   uint64_t steps;
   float lr, beta1, beta2, eps;
   uint64_t offset; /* first row of the first minibatch                 */
+  const float* weights; /* optional device [n x n_out] loss weights: the
+                           minimised loss is sum w (f - y)^2 / (2 b n_out) */
 } mmc_train_opts;
 int mmc_mlp_train(mmc_ctx* ctx, uint64_t n, const float* x, const float* y,
                   const mmc_train_opts* opts, float* last_loss);

@@ -201,7 +201,7 @@ class MlpDesc(C.Structure):
 class TrainOpts(C.Structure):
     _fields_ = [("batch", C.c_uint64), ("steps", C.c_uint64), ("lr", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
-                ("offset", C.c_uint64)]
+                ("offset", C.c_uint64), ("weights", C.c_void_p)]
 
 
 STRUCTS = {

@@ -782,14 +782,15 @@ __global__ void KSiluBack(const float* Z, const float* dH, float* dZ, size_t n) 
   }
 }
 // dZ = (Z - Y) / (rows * outs); loss partial sums
-__global__ void KLossGrad(const float* Z, const float* Y, float* dZ, size_t n, float inv,
-                          float* loss) {
+__global__ void KLossGrad(const float* Z, const float* Y, const float* w, float* dZ, size_t n,
+                          float inv, float* loss) {
   float acc = 0.0f;
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
        i += size_t(gridDim.x) * blockDim.x) {
     const float d = Z[i] - Y[i];
-    dZ[i] = d * inv;
-    acc += d * d;
+    const float wd = w ? w[i] * d : d;
+    dZ[i] = wd * inv;
+    acc += wd * d;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if (threadIdx.x % 64 == 0) atomicAdd(loss, acc * 0.5f * inv);
@@ -962,7 +963,8 @@ extern "C" int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x
 }
 
 // Forward + backward of one minibatch (normalised x, y) into m.grads.
-static int ForwardBackward(mmc_ctx* ctx, uint64_t rows, const float* x, const float* y) {
+static int ForwardBackward(mmc_ctx* ctx, uint64_t rows, const float* x, const float* y,
+                           const float* w = nullptr) {
   MlpState& m = MmcMlp(ctx);
   hipStream_t s = MmcStream(ctx);
   const int L = m.n_layers;
@@ -998,8 +1000,8 @@ static int ForwardBackward(mmc_ctx* ctx, uint64_t rows, const float* x, const fl
   hipLaunchKernelGGL(KZero, dim3(256), dim3(256), 0, s, m.grads, size_t(m.n_params));
   hipLaunchKernelGGL(KZero, dim3(1), dim3(64), 0, s, m.loss, size_t(1));
   const int no = m.widths[L];
-  hipLaunchKernelGGL(KLossGrad, dim3(1024), dim3(256), 0, s, Z[L - 1], y, dZ, size_t(R) * no,
-                     1.0f / float(R * no), m.loss);
+  hipLaunchKernelGGL(KLossGrad, dim3(1024), dim3(256), 0, s, Z[L - 1], y, w, dZ,
+                     size_t(R) * no, 1.0f / float(R * no), m.loss);
   for (int l = L - 1; l >= 0; l--) {
     const int nout = m.widths[l + 1], nin = m.widths[l];
     const float* in = l == 0 ? x : H[l - 1];
@@ -1054,7 +1056,8 @@ extern "C" int mmc_mlp_train(mmc_ctx* ctx, uint64_t n, const float* x, const flo
   const int ni = m.widths[0], no = m.widths[m.n_layers];
   for (uint64_t step = 0; step < o->steps; step++) {
     if (off + o->batch > n) off = 0;
-    int rc = ForwardBackward(ctx, o->batch, x + off * ni, y + off * no);
+    int rc = ForwardBackward(ctx, o->batch, x + off * ni, y + off * no,
+                             o->weights ? o->weights + off * no : nullptr);
     if (rc) return rc;
     m.adam_t++;
     const float c1 = 1.0f - std::pow(o->beta1, float(m.adam_t));

@@ -11,6 +11,7 @@ energy transfer leaves the kinematic support (E' <= 0), the tracker falls back
 to POD.
 """
 import ctypes as C
+import hashlib
 import os
 
 import numpy as np
@@ -45,12 +46,56 @@ def make_desc(out_shift=(0.0, 0.5), out_scale=(1.0, 0.3), widths=WIDTHS,
     return d
 
 
+# Loss weights: the judge of the sampler is per (E, T) (Wasserstein distance
+# relative to the POD marginal's spread there), and that spread varies about
+# 3x over the box (the beta target is narrow above a few kT).  Training
+# therefore weights each sample by var_global / var_bin of its target, with
+# variances per bin of a WEIGHT_BINS (ln E x T) grid measured on the first
+# chunk of targets, clipped to WEIGHT_CLIP and normalised to mean 1.
+WEIGHT_BINS = (32, 8)
+WEIGHT_CLIP = (0.2, 20.0)
+
+
+def weight_bins(x):
+    nE, nT = WEIGHT_BINS
+    fe = (x[:, 0] - np.log(E_LO)) / (np.log(E_HI) - np.log(E_LO))
+    ft = (x[:, 1] - T_LO) / (T_HI - T_LO)
+    ie = np.clip((fe * nE).astype(np.int64), 0, nE - 1)
+    it = np.clip((ft * nT).astype(np.int64), 0, nT - 1)
+    return ie * nT + it
+
+
+def weight_table(x, y):
+    """[bins x 2] weights var_global / var_bin from raw inputs and targets."""
+    b = weight_bins(x)
+    nb = WEIGHT_BINS[0] * WEIGHT_BINS[1]
+    cnt = np.maximum(np.bincount(b, minlength=nb), 1)
+    tab = np.ones((nb, 2))
+    for k in range(2):
+        m1 = np.bincount(b, y[:, k], minlength=nb) / cnt
+        m2 = np.bincount(b, y[:, k].astype(np.float64) ** 2, minlength=nb) / cnt
+        var = np.maximum(m2 - m1 * m1, 1e-12)
+        tab[:, k] = np.clip(y[:, k].var() / var, *WEIGHT_CLIP)
+        tab[:, k] /= (tab[b, k]).mean()
+    return tab
+
+
+# Fraction of training energies drawn uniformly in E (the rest log-uniform):
+# the log-uniform draw alone leaves few samples in the upper decade, where
+# the beta law is widest.
+E_UNIFORM_FRACTION = 0.3
+
+
 def sample_inputs(n, seed):
-    """Raw inputs: E log-uniform on [1e-5, 4) eV, T uniform on [300, 1200] K,
-    u1, u2 uniform on [0, 1)."""
+    """Raw inputs: E from a mixture of log-uniform (1 - E_UNIFORM_FRACTION)
+    and uniform (E_UNIFORM_FRACTION) on [1e-5, 4) eV, T uniform on
+    [300, 1200] K, u1, u2 uniform on [0, 1)."""
     rng = np.random.default_rng(seed)
     x = np.empty((n, 4), dtype=np.float32)
-    x[:, 0] = rng.uniform(np.log(E_LO), np.log(E_HI), n)
+    lnE = rng.uniform(np.log(E_LO), np.log(E_HI), n)
+    lin = rng.random(n) < E_UNIFORM_FRACTION
+    lnE[lin] = np.log(rng.uniform(E_LO, E_HI, int(lin.sum())))
+    x[:, 0] = lnE
     x[:, 1] = rng.uniform(T_LO, T_HI, n)
     x[:, 2] = rng.uniform(0.0, 1.0, n)
     x[:, 3] = rng.uniform(0.0, 1.0, n)
@@ -102,13 +147,15 @@ class Surrogate:
         return y
 
     def train(self, n_samples=1 << 26, batch=8192, epochs=3, lr=2e-3, seed=11,
-              chunk=1 << 24, log=None):
-        """Generates n_samples POD targets on the device and trains."""
+              chunk=1 << 24, log=None, weighted=True):
+        """Generates n_samples POD targets on the device and trains (with the
+        per-(E, T) loss weights above unless weighted is False)."""
         n = n_samples
         x = self.t.device_array(shape=(n, 4), dtype=np.float32)
         y = self.t.device_array(shape=(n, 2), dtype=np.float32)
+        w = self.t.device_array(shape=(n, 2), dtype=np.float32) if weighted else None
         # inputs in chunks (host RNG), targets from the POD map on the device
-        stats = []
+        stats, wtab = [], None
         for c0 in range(0, n, chunk):
             m = min(chunk, n - c0)
             xc = sample_inputs(m, seed + c0)
@@ -119,6 +166,11 @@ class Surrogate:
                 yc = np.empty((m, 2), dtype=np.float32)
                 lib().mmc_memcpy_d2h(self.t.h, yc.ctypes.data, y.ptr, yc.nbytes)
                 stats = (yc.mean(0), yc.std(0) + 1e-6)
+                if weighted:
+                    wtab = weight_table(xc, yc)
+            if weighted:
+                wc = np.ascontiguousarray(wtab[weight_bins(xc)], dtype=np.float32)
+                lib().mmc_memcpy_h2d(self.t.h, w.ptr + c0 * 8, wc.ctypes.data, wc.nbytes)
         desc = make_desc(out_shift=tuple(float(v) for v in stats[0]),
                          out_scale=tuple(float(v) for v in stats[1]))
         self.set(None, desc)
@@ -135,6 +187,7 @@ class Surrogate:
                 o.lr = float(lr * 0.5 * (1 + np.cos(np.pi * frac)) + 1e-5)
                 o.beta1, o.beta2, o.eps = 0.9, 0.999, 1e-8
                 o.offset = (part * (steps_per_epoch // 8)) * batch
+                o.weights = w.ptr if weighted else None
                 loss = C.c_float()
                 self._check(lib().mmc_mlp_train(self.t.h, n, x.data_ptr(), y.data_ptr(),
                                                 C.byref(o), C.byref(loss)))
@@ -143,29 +196,70 @@ class Surrogate:
                     log("epoch %d part %d lr %.2e loss %.3e" % (ep, part, o.lr, loss.value))
         x.free()
         y.free()
+        if w is not None:
+            w.free()
         return losses
 
-    def save(self, path):
+    def save(self, path, info=None):
+        """Writes the network and the digest of the thermal-scattering data
+        it was trained on (plus free-form training information)."""
         d = self.desc
         np.savez(path, params=self.params(), in_shift=np.array(d.in_shift),
                  in_scale=np.array(d.in_scale), out_shift=np.array(d.out_shift),
-                 out_scale=np.array(d.out_scale))
+                 out_scale=np.array(d.out_scale),
+                 tables_digest=np.array(tables_digest(self.t.problem.tsl[self.tsl])),
+                 info=np.array(repr(info or {})))
 
     def load(self, path):
-        z = np.load(path)
+        """Loads a saved network; raises ValueError if it was trained on other
+        thermal-scattering data than this tracker's."""
+        z = np.load(path, allow_pickle=False)
+        want = tables_digest(self.t.problem.tsl[self.tsl])
+        if "tables_digest" not in z or str(z["tables_digest"]) != want:
+            raise ValueError("%s was trained on other thermal-scattering data" % path)
         desc = make_desc(out_shift=tuple(z["out_shift"]), out_scale=tuple(z["out_scale"]))
         self.set(z["params"], desc)
 
 
-def cached_network(tracker, tsl=0, path=None, log=None, **train_kw):
-    """Loads the trained network from minimc_amd/_build, training it first
-    when it is not there."""
-    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build",
-                                "tsl_surrogate.npz")
-    s = Surrogate(tracker, tsl)
-    if os.path.exists(path):
-        s.load(path)
+def _feed(h, v):
+    if isinstance(v, dict):
+        for k in sorted(v, key=str):
+            h.update(str(k).encode())
+            _feed(h, v[k])
+    elif isinstance(v, (list, tuple)):
+        for a in v:
+            _feed(h, a)
+    elif isinstance(v, np.ndarray):
+        h.update(np.ascontiguousarray(v).tobytes())
     else:
-        s.train(log=log, **train_kw)
-        s.save(path)
-    return s
+        h.update(repr(v).encode())
+
+
+def tables_digest(t):
+    """SHA-1 of a thermal-scattering table set (nested dicts/lists of arrays
+    and numbers)."""
+    h = hashlib.sha1()
+    _feed(h, t)
+    return h.hexdigest()
+
+
+# The network trained for the synthetic thermal-scattering tables of the
+# benchmark configurations (configs.tsl_tables()), shipped with the package.
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tsl_surrogate.npz")
+
+
+def cached_network(tracker, tsl=0, path=None, log=None, **train_kw):
+    """Loads the network at `path` (default: the shipped one), training it
+    on the device first when it is missing or was trained on other tables.
+    Returns (Surrogate, trained_now)."""
+    path = path or SHIPPED
+    s = Surrogate(tracker, tsl)
+    try:
+        s.load(path)
+        return s, False
+    except (OSError, ValueError):
+        pass
+    s.train(log=log, **train_kw)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    s.save(path, info=dict(train_kw))
+    return s, True

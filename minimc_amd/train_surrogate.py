@@ -72,7 +72,7 @@ def main():
     pr = configs.tsl_box(configs.tsl_tables(), T=600.0)
     t = Tracker(pr, max_in_flight=1 << 16)
     s = surrogate.Surrogate(t)
-    out = args.out or os.path.join(ROOT, "minimc_amd", "_build", "tsl_surrogate.npz")
+    out = args.out or surrogate.SHIPPED
     info = {}
     if args.eval_only:
         s.load(out)
@@ -81,11 +81,12 @@ def main():
         losses = s.train(n_samples=1 << args.samples, batch=args.batch, epochs=args.epochs,
                          lr=args.lr, log=lambda m: print(m, flush=True))
         dt = time.perf_counter() - t0
-        s.save(out)
         steps = args.epochs * ((1 << args.samples) // args.batch)
         info = dict(train_seconds=dt, steps=steps, samples=1 << args.samples,
+                    epochs=args.epochs, batch=args.batch, lr=args.lr,
                     final_loss=losses[-1],
                     train_tflops=steps * args.batch * 3 * 2 * surrogate.n_params() / dt / 1e12)
+        s.save(out, info=info)
     rows = judge(t)
     for r in rows:
         print("E=%6.3f eV T=%6.1f  W1 beta %.4f alpha %.4f  KS beta %.4f alpha %.4f" % (

@@ -41,3 +41,22 @@ def test_input_sampling_box():
     assert x[:, 0].min() >= np.log(1e-11) - 1e-5 and x[:, 0].max() <= np.log(4e-6) + 1e-5
     assert x[:, 1].min() >= 300 and x[:, 1].max() <= 1200
     assert 0 <= x[:, 2].min() and x[:, 3].max() < 1
+
+
+def test_loss_weights_equalise_bin_variance():
+    """Training weights w = var_global / var_bin (clipped, mean 1): on a
+    target whose spread differs 4x between temperature halves, w * var_bin
+    is the same in every bin."""
+    from minimc_amd import surrogate
+    x = surrogate.sample_inputs(400000, 3)
+    rng = np.random.default_rng(4)
+    scale = np.where(x[:, 1] > 750.0, 4.0, 1.0)
+    y = np.stack([rng.normal(0, 1, len(x)) * scale, rng.normal(0, 1, len(x))], 1)
+    tab = surrogate.weight_table(x, y.astype(np.float32))
+    b = surrogate.weight_bins(x)
+    assert np.allclose(tab[b].mean(0), 1.0)
+    cnt = np.bincount(b)
+    var0 = np.bincount(b, y[:, 0] ** 2) / cnt
+    prod = tab[:, 0] * var0
+    assert prod.max() / prod.min() < 1.1  # equalised (sampling noise only)
+    assert tab[:, 1].max() / tab[:, 1].min() < 1.35  # nothing to equalise (noise only)
