@@ -450,8 +450,8 @@ int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
  * MLP (log E, T, u1, u2) -> (alpha, beta), widths w[0..n_layers], SiLU
  * between layers.  Parameters are FP32 masters; inference runs BF16 MFMA
  * with FP32 accumulation.  Inputs outside [e_min <= E <= e_max,
- * t_lo <= T <= t_hi], and network outputs outside the kinematic support
- * (E' < 0), fall back to the POD sampler. */
+ * t_lo <= T <= t_hi], and network outputs whose outgoing energy leaves the
+ * box (E' < e_min, including E' <= 0), fall back to the POD sampler. */
 typedef struct mmc_mlp_desc {
   int32_t n_layers; /* number of linear layers                         */
   int32_t widths[8];

@@ -751,29 +751,43 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
 // ---- bank compaction ----------------------------------------------------------
 // As histories end, the live particles spread over ever more of the bank, and
 // a wave's loads of one field touch one cache line per lane.  This kernel
-// moves the live queue's particles to slots 0..n-1 of the other bank (the
-// host then swaps the banks), so that the following iterations read the
-// particle state with unit stride.  The k-eigenvalue sums are indexed by
-// history and stay where they are; tslT/tslI are only live within one
-// iteration.
-__global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin) {
+// moves the live queue's particles to slots 0..n-1 of the other bank, and
+// the particles parked for the POD sampler by the surrogate step (fb, length
+// misc[kMiscFallback]) to the slots after them; the host then swaps the
+// banks, so that the following iterations read the particle state with unit
+// stride.  The k-eigenvalue sums are indexed by history and stay where they
+// are.
+__device__ __forceinline__ void MoveParticle(const DevBank& src, uint32_t p, const DevBank& dst,
+                                             uint32_t i) {
+  dst.x[i] = src.x[p]; dst.y[i] = src.y[p]; dst.z[i] = src.z[p];
+  dst.u[i] = src.u[p]; dst.v[i] = src.v[p]; dst.w[i] = src.w[p];
+  dst.E[i] = src.E[p];
+  dst.rng[i] = src.rng[p];
+  dst.cell[i] = src.cell[p];
+  dst.surf[i] = src.surf[p];
+  dst.hist[i] = src.hist[p];
+  dst.steps[i] = src.steps[p];
+  dst.xs_grid[i] = src.xs_grid[p];
+  dst.xs_gi[i] = src.xs_gi[p];
+  dst.tslT[i] = src.tslT[p];
+  dst.tslI[i] = src.tslI[p];
+}
+__global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin,
+                                                uint32_t* fb) {
   const uint32_t n = *Ring(R, iter - 1, kQ);
+  const uint32_t nf = fb ? R.misc[kMiscFallback] : 0u;
   uint32_t* Q = R.Q[qin];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t p = Q[i];
-    dst.x[i] = R.B.x[p]; dst.y[i] = R.B.y[p]; dst.z[i] = R.B.z[p];
-    dst.u[i] = R.B.u[p]; dst.v[i] = R.B.v[p]; dst.w[i] = R.B.w[p];
-    dst.E[i] = R.B.E[p];
-    dst.rng[i] = R.B.rng[p];
-    dst.cell[i] = R.B.cell[p];
-    dst.surf[i] = R.B.surf[p];
-    dst.hist[i] = R.B.hist[p];
-    dst.steps[i] = R.B.steps[p];
-    dst.xs_grid[i] = R.B.xs_grid[p];
-    dst.xs_gi[i] = R.B.xs_gi[p];
-    Q[i] = i;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n + nf;
+       i += gridDim.x * blockDim.x) {
+    if (i < n) {
+      MoveParticle(R.B, Q[i], dst, i);
+      Q[i] = i;
+    } else {
+      MoveParticle(R.B, fb[i - n], dst, i);
+      fb[i - n] = i;
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n;  // slots in use
+  if (blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n + nf;  // slots in use
 }
 
 // ---- tail: history-based completion of the last live particles ------------

@@ -131,15 +131,19 @@ __device__ __forceinline__ void MfmaLayer(const bf16x8* lds, int lane, const bf1
   }
 }
 
-// Returns false when the network's energy transfer leaves the kinematic
-// support (E' = E + beta kT <= 0, or not a number): the caller then uses the
-// POD sampler for that collision.
+// Returns false when the network's outgoing energy leaves the trained box
+// (E' = E + beta kT below e_min, which includes E' <= 0, or not a number):
+// the caller then uses the POD sampler for that collision.  A network that
+// undershoots the lower end of the beta law (E' -> 0) would otherwise send
+// particles to energies where the POD sampler's rejection loop
+// (ThermalScattering.cpp:302-337) runs out of resamples.
 __device__ __forceinline__ bool OutputsToAlphaBeta(double y1, double y2, double E, double T,
                                                    double awr, double beta_cutoff,
-                                                   double& alpha, double& beta) {
+                                                   double e_min, double& alpha,
+                                                   double& beta) {
   const double kT = kBoltzmann * T;
   double b = y1 * (E + kT) / kT;
-  if (!(E + b * kT > 0)) return false;
+  if (!(E + b * kT >= e_min) || !(E + b * kT > 0)) return false;
   if (b > beta_cutoff) b = beta_cutoff;
   const double Ep = E + b * kT;
   const double sE = sqrt(E), sp = sqrt(Ep > 0 ? Ep : 0.0);
@@ -300,7 +304,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
       const DTsl t = io.W.tsl[R.B.tslI[p]];
       double a = 0, b = 0;
       if (InBoxT(io, E, T) &&
-          OutputsToAlphaBeta(y1, y2, E, T, t.awr, t.beta_cutoff, a, b)) {
+          OutputsToAlphaBeta(y1, y2, E, T, t.awr, t.beta_cutoff, io.e_min, a, b)) {
         V3 dir = LoadDir(R.B, p);
         uint32_t g = R.B.rng[p];
         const double E_p = E + b * kBoltzmann * T;
@@ -1103,7 +1107,7 @@ __global__ void KSurrogateFinish(DevWorld W, MlpState m, int tsl, uint32_t n, co
   double a, b;
   if (!InBox(m, E[i], T[i]) ||
       !OutputsToAlphaBeta(yout[size_t(i) * 2], yout[size_t(i) * 2 + 1], E[i], T[i], t.awr,
-                          t.beta_cutoff, a, b)) {
+                          t.beta_cutoff, m.e_min, a, b)) {
     uint32_t g = rng[i];
     uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
     if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);

@@ -497,7 +497,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   B.xs_grid = DevAlloc<int32_t>(ctx, cap);
   B.xs_gi = DevAlloc<int32_t>(ctx, cap);
   // the compaction target: the moved fields get their own arrays, the
-  // history-indexed and transient ones are shared
+  // history-indexed k sums are shared
   DevBank& A = ctx->alt;
   A = B;
   A.x = DevAlloc<double>(ctx, cap);
@@ -514,6 +514,8 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   A.steps = DevAlloc<uint32_t>(ctx, cap);
   A.xs_grid = DevAlloc<int32_t>(ctx, cap);
   A.xs_gi = DevAlloc<int32_t>(ctx, cap);
+  A.tslT = DevAlloc<double>(ctx, cap);
+  A.tslI = DevAlloc<int32_t>(ctx, cap);
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
@@ -540,7 +542,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
   if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x || !A.y || !A.z || !A.u ||
       !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.surf || !A.hist || !A.steps ||
-      !A.xs_grid || !A.xs_gi)
+      !A.xs_grid || !A.xs_gi || !A.tslT || !A.tslI)
     return Fail(ctx, MMC_ERR_HIP, "out of device memory");
   HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
   R.max_steps = uint32_t(ctx->opt.max_steps ? std::min<uint64_t>(ctx->opt.max_steps, 0xffffffffull)
@@ -839,6 +841,8 @@ static void SwapBanks(mmc_ctx* ctx) {
   std::swap(B.steps, A.steps);
   std::swap(B.xs_grid, A.xs_grid);
   std::swap(B.xs_gi, A.xs_gi);
+  std::swap(B.tslT, A.tslT);
+  std::swap(B.tslI, A.tslI);
 }
 
 // Runs event iterations until no particle is alive.  The live-queue length
@@ -888,7 +892,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     for (int k = 0; k < K; k++) {
       if (compact_ok && every > 0 && iter > 0 && iter % every == 0) {
         hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
-                           ctx->alt, iter, qin);
+                           ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr);
         SwapBanks(ctx);
       }
       const int rc = LaunchIteration(ctx, iter, qin, grid);
@@ -916,7 +920,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
     if (compact_ok && every == 0 && 2 * live < in_use) {
       hipLaunchKernelGGL(KCompact, dim3(Blocks(live)), dim3(kBlock), 0, ctx->stream, ctx->R,
-                         ctx->alt, iter, qin);
+                         ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr);
       SwapBanks(ctx);
       in_use = live;
     }
@@ -1087,23 +1091,49 @@ __global__ void KResample(const mmc_site* bank, uint64_t M, uint64_t N, mmc_site
   s.key = j;
   src[j] = s;
 }
+// Entropy-mesh counts of the banked sites.  Meshes of up to kMeshLds bins
+// are counted in a per-block LDS histogram (the bank is ordered by history,
+// so neighbouring sites hit the same bins and global atomics would serialise
+// on them), merged with one global atomic per non-empty bin and block; the
+// grid is kept small (kMeshBlocks) so that the merge stays cheap.
+constexpr int kMeshLds = 8192;
+constexpr uint32_t kMeshBlocks = 128;
 __global__ void KMeshCount(const mmc_site* s, uint32_t m, int nx, int ny, int nz, double lx,
                            double ly, double lz, double hx, double hy, double hz,
-                           uint32_t* cnt) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const double q[3] = {s[k].x, s[k].y, s[k].z};
+                           uint32_t* cnt, int nmesh) {
+  extern __shared__ uint32_t hist_lds[];
+  const bool lds = nmesh <= kMeshLds;
+  if (lds) {
+    for (int i = threadIdx.x; i < nmesh; i += blockDim.x) hist_lds[i] = 0;
+    __syncthreads();
+  }
   const double lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
   const int nn[3] = {nx, ny, nz};
-  int idx = 0;
-  for (int a = 0; a < 3; a++) {
-    const double f = (q[a] - lo[a]) / (hi[a] - lo[a]);
-    if (!(f >= 0 && f < 1)) return;
-    int i = int(f * nn[a]);
-    if (i >= nn[a]) i = nn[a] - 1;
-    idx = idx * nn[a] + i;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+    const double q[3] = {s[k].x, s[k].y, s[k].z};
+    int idx = 0;
+    bool in = true;
+    for (int a = 0; a < 3 && in; a++) {
+      const double f = (q[a] - lo[a]) / (hi[a] - lo[a]);
+      if (!(f >= 0 && f < 1)) {
+        in = false;
+      } else {
+        int i = int(f * nn[a]);
+        if (i >= nn[a]) i = nn[a] - 1;
+        idx = idx * nn[a] + i;
+      }
+    }
+    if (!in) continue;
+    if (lds)
+      atomicAdd(hist_lds + idx, 1u);
+    else
+      atomicAdd(cnt + idx, 1u);
   }
-  atomicAdd(cnt + idx, 1u);
+  if (lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nmesh; i += blockDim.x)
+      if (hist_lds[i]) atomicAdd(cnt + i, hist_lds[i]);
+  }
 }
 // deterministic per-block sums of the per-history k accumulators
 __global__ void KKSum(const double* a, const double* b, const double* c, uint32_t n,
@@ -1156,10 +1186,14 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   const uint32_t nmesh = uint32_t(ctx->mesh_n[0]) * ctx->mesh_n[1] * ctx->mesh_n[2];
   if (nmesh > (1u << 20)) return Fail(ctx, MMC_ERR_ARG, "entropy mesh too large");
   FillU32(ctx, ctx->mesh_cnt, 0, nmesh);
-  if (M) hipLaunchKernelGGL(KMeshCount, dim3(Blocks(M, 1u << 30)), dim3(kBlock), 0,
-                            ctx->stream, ctx->sites_sorted, M, ctx->mesh_n[0], ctx->mesh_n[1],
-                            ctx->mesh_n[2], ctx->mesh_lo[0], ctx->mesh_lo[1], ctx->mesh_lo[2],
-                            ctx->mesh_hi[0], ctx->mesh_hi[1], ctx->mesh_hi[2], ctx->mesh_cnt);
+  if (M) {
+    const size_t lds = nmesh <= uint32_t(kMeshLds) ? sizeof(uint32_t) * nmesh : 0;
+    hipLaunchKernelGGL(KMeshCount, dim3(Blocks(M, lds ? kMeshBlocks : 1u << 30)), dim3(kBlock),
+                       lds, ctx->stream, ctx->sites_sorted, M, ctx->mesh_n[0], ctx->mesh_n[1],
+                       ctx->mesh_n[2], ctx->mesh_lo[0], ctx->mesh_lo[1], ctx->mesh_lo[2],
+                       ctx->mesh_hi[0], ctx->mesh_hi[1], ctx->mesh_hi[2], ctx->mesh_cnt,
+                       int(nmesh));
+  }
   const uint32_t nbk = Blocks(N, 1u << 30);
   hipLaunchKernelGGL(KKSum, dim3(nbk), dim3(kBlock), 0, ctx->stream, R.B.k_col, R.B.k_trk,
                      R.B.k_abs, N, ctx->kpart);

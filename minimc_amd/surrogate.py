@@ -5,10 +5,10 @@ position of alpha in [alpha_min(beta), alpha_max(beta)]).  Its training
 targets come from the POD tables themselves: `mmc_tsl_pod_map` evaluates the
 POD inverse CDFs of beta (level u1) and alpha (level u2) on the device.  The
 forward and backward passes and the Adam steps are the library's own FP32
-kernels; inference is the fused BF16-MFMA kernel.  Outside the trained box
+kernels; inference is two BF16-MFMA kernels.  Outside the trained box
 (E outside [e_min, e_max] or T outside [t_lo, t_hi]) and where the network's
-energy transfer leaves the kinematic support (E' <= 0), the tracker falls back
-to POD.
+outgoing energy leaves the box (E' < e_min, including E' <= 0), the tracker
+falls back to POD.
 """
 import ctypes as C
 import hashlib
