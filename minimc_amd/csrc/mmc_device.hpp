@@ -61,8 +61,9 @@ struct DComp {
 };
 // A ContinuousMap on a (possibly shared) key grid.
 struct DTable {
-  int grid;   // index into grids
-  int y_off;  // values in pool
+  int grid;      // index into grids
+  int y_off;     // values in pool: value i at pool[y_off + i * y_stride]
+  int y_stride;  // 1, or the column count of an interleaved block
 };
 struct DGrid {
   int x_off, n;         // keys in pool
@@ -341,24 +342,25 @@ __device__ __forceinline__ int GridUpperBound(const DevWorld& W, const DGrid& g,
   return UpperBoundRange(x, lo, hi, key);
 }
 __device__ __forceinline__ double TableAtIndex(const DevWorld& W, const DGrid& g,
-                                               int y_off, int hi, double k) {
+                                               const DTable& tb, int hi, double k) {
   const double* x = W.pool + g.x_off;
-  const double* y = W.pool + y_off;
-  if (hi == g.n) return y[hi - 1];
+  const double* y = W.pool + tb.y_off;
+  const int s = tb.y_stride;
+  if (hi == g.n) return y[(hi - 1) * s];
   if (hi == 0) return y[0];
-  const double k_hi = x[hi], v_hi = y[hi], k_lo = x[hi - 1], v_lo = y[hi - 1];
+  const double k_hi = x[hi], v_hi = y[hi * s], k_lo = x[hi - 1], v_lo = y[(hi - 1) * s];
   return v_lo + (v_hi - v_lo) / (k_hi - k_lo) * (k - k_lo);
 }
 __device__ __forceinline__ double TableAt(const DevWorld& W, int t, double k) {
   const DTable tb = W.tables[t];
   const DGrid g = W.grids[tb.grid];
   const int hi = GridUpperBound(W, g, W.ipool + g.hash_off, k);
-  return TableAtIndex(W, g, tb.y_off, hi, k);
+  return TableAtIndex(W, g, tb, hi, k);
 }
 // With a precomputed grid index (same grid).
 __device__ __forceinline__ double TableAtI(const DevWorld& W, int t, int hi, double k) {
   const DTable tb = W.tables[t];
-  return TableAtIndex(W, W.grids[tb.grid], tb.y_off, hi, k);
+  return TableAtIndex(W, W.grids[tb.grid], tb, hi, k);
 }
 
 // ---- thermal scattering POD (src/ThermalScattering.cpp) -------------------

@@ -151,6 +151,7 @@ struct Flat {
     DTable d{};
     d.grid = AddGrid(t.x, t.n);
     d.y_off = AddDoubles(t.y, t.n);
+    d.y_stride = 1;
     tables.push_back(d);
     return int(tables.size()) - 1;
   }
@@ -326,6 +327,23 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
         rx.push_back(dr);
       }
       dn.grid = grid >= 0 ? grid : -1;
+      // A collision reads every reaction's cross section at the same grid
+      // index: interleave the reaction tables of a shared grid point by
+      // point, so that one cache line serves all of them.
+      if (dn.grid >= 0 && dn.rx_count >= 2) {
+        const int c = dn.rx_count, n = F.grids[dn.grid].n;
+        std::vector<double> block(size_t(n) * c);
+        for (int k = 0; k < c; k++) {
+          const DTable& tb = F.tables[rx[dn.rx_begin + k].xs];
+          for (int i = 0; i < n; i++) block[size_t(i) * c + k] = F.pool[tb.y_off + i];
+        }
+        const int off = F.AddDoubles(block.data(), block.size());
+        for (int k = 0; k < c; k++) {
+          DTable& tb = F.tables[rx[dn.rx_begin + k].xs];
+          tb.y_off = off + k;
+          tb.y_stride = c;
+        }
+      }
       ce.push_back(dn);
     }
     for (int i = 0; i < wd->n_tsl; i++) {
