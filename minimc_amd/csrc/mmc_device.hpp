@@ -82,6 +82,10 @@ struct DCeNuclide {
   int grid;              // the grid shared by every table of this nuclide, or -1
   int fissile;
 };
+// The sampler's axes and CDF mode tables (F_off, cdf_off, S_off, T_off of a
+// partition; Es_off, E_ends_off, betas_off, beta_ends_off of a table) are
+// offsets into DevWorld::hot, a block small enough to be copied into LDS by
+// the kernels that sample.  x_off and xt_off are offsets into the pool.
 struct DTslPart {
   int F_off, nF, cdf_off, S_off, x_off, nx, T_off, nT, xt_off, rank;
 };
@@ -90,6 +94,7 @@ struct DTsl {
   int E_axis_off, nE, E_modes_off, T_axis_off, nT, T_modes_off, S_off, rank;
   int Es_off, nEs, E_ends_off, n_bp, bp_begin;       // beta partitions
   int betas_off, nbetas, beta_ends_off, n_ap, ap_begin;  // alpha partitions
+  // (E_ends and beta_ends are stored as doubles; they are small integers)
   int pad;
   double beta_cutoff, alpha_cutoff, awr, cutoff_energy;
 };
@@ -129,6 +134,8 @@ struct DevWorld {
   const double* pool;
   const int* ipool;
   const DEst* ests;
+  const double* hot;  // thermal-scattering sampler tables (see DTsl)
+  int hot_n, pad_h;   // doubles in hot
 };
 
 // ---- RNG: std::minstd_rand with libstdc++ canonical (see oracle/rng.hpp) --
@@ -373,9 +380,9 @@ __device__ __forceinline__ int UpperBoundArr(const double* x, int n, double key)
 struct TBracket {
   int t_lo, t_hi;
 };
-__device__ __forceinline__ TBracket PartTBracket(const DevWorld& W, const DTslPart& P,
+__device__ __forceinline__ TBracket PartTBracket(const double* H, const DTslPart& P,
                                                  double T) {
-  const double* Ts = W.pool + P.T_off;
+  const double* Ts = H + P.T_off;
   const int cand = UpperBoundArr(Ts, P.nT, T);
   const bool above = cand == P.nT;
   const int t_hi = above ? cand - 1 : cand;
@@ -383,13 +390,13 @@ __device__ __forceinline__ TBracket PartTBracket(const DevWorld& W, const DTslPa
   return {below ? t_hi : t_hi - 1, t_hi};
 }
 // BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256)
-__device__ __forceinline__ double PartEvaluateAt(const DevWorld& W, const DTslPart& P,
-                                                 int cdf_i, int x_i, double T,
-                                                 TBracket br) {
-  const double* Ts = W.pool + P.T_off;
+__device__ __forceinline__ double PartEvaluateAt(const DevWorld& W, const double* H,
+                                                 const DTslPart& P, int cdf_i, int x_i,
+                                                 double T, TBracket br) {
+  const double* Ts = H + P.T_off;
   const int t_hi = br.t_hi, t_lo = br.t_lo;
-  const double* S = W.pool + P.S_off;
-  const double* cdf = W.pool + P.cdf_off + cdf_i * P.rank;
+  const double* S = H + P.S_off;
+  const double* cdf = H + P.cdf_off + cdf_i * P.rank;
   const double* xh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
   const double* xl = W.pool + P.xt_off + (x_i * P.nT + t_lo) * P.rank;
   double v_hi = 0, v_lo = 0;
@@ -401,9 +408,10 @@ __device__ __forceinline__ double PartEvaluateAt(const DevWorld& W, const DTslPa
   const double T_hi = Ts[t_hi], T_lo = Ts[t_lo];
   return v_lo + (v_hi - v_lo) / (T_hi - T_lo) * (T - T_lo);
 }
-__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const DTslPart& P,
-                                               int cdf_i, int x_i, double T) {
-  return PartEvaluateAt(W, P, cdf_i, x_i, T, PartTBracket(W, P, T));
+__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const double* H,
+                                               const DTslPart& P, int cdf_i, int x_i,
+                                               double T) {
+  return PartEvaluateAt(W, H, P, cdf_i, x_i, T, PartTBracket(H, P, T));
 }
 // ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
 __device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, double E,
@@ -439,30 +447,38 @@ __device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, dou
   return xs_T_lo + r_T * (xs_T_hi - xs_T_lo);
 }
 // ThermalScattering::SampleBeta (ThermalScattering.cpp:271-338)
-__device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uint32_t& g,
-                                           double E, double T, double& beta) {
-  const double* Es = W.pool + t.Es_off;
+// Partition of a concatenated grid index (beta_partition_E_ends /
+// alpha_partition_beta_ends) and the index local to it.
+__device__ __forceinline__ int PartitionOf(const double* ends, int n_parts, int idx,
+                                           int& local) {
+  int p_i = 0;
+  while (p_i < n_parts && ends[p_i] <= idx) p_i++;
+  local = p_i == 0 ? idx : idx - int(ends[p_i - 1]);
+  return p_i;
+}
+// H: the hot block (DevWorld::hot or its copy in LDS).
+__device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, const DTsl& t,
+                                           uint32_t& g, double E, double T, double& beta) {
+  const double* Es = H + t.Es_off;
   const int e_hi = UpperBoundArr(Es, t.nEs, E);
   if (e_hi >= t.nEs) return false;  // E above the beta grid (assert at :278)
   const double r = e_hi != 0 ? (E - Es[e_hi - 1]) / (Es[e_hi] - Es[e_hi - 1]) : 1;
   const int e_s = r <= Canonical(g) ? e_hi - 1 : e_hi;
   const double E_s = Es[e_s];
-  const int* ends = W.ipool + t.E_ends_off;
-  int p_i = 0;
-  while (p_i < t.n_bp && ends[p_i] <= e_s) p_i++;
+  int local;
+  const int p_i = PartitionOf(H + t.E_ends_off, t.n_bp, e_s, local);
   const DTslPart P = W.parts[t.bp_begin + p_i];
-  const int local = p_i == 0 ? e_s : e_s - ends[p_i - 1];
-  const double* F = W.pool + P.F_off;
-  const TBracket br = PartTBracket(W, P, T);
+  const double* F = H + P.F_off;
+  const TBracket br = PartTBracket(H, P, T);
   for (int k = 0; k < kBetaResampleLimit; k++) {
     const double u = Canonical(g);
     const int f_hi = UpperBoundArr(F, P.nF, u);
     const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
     const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-    const double b_lo = f_hi != 0 ? PartEvaluateAt(W, P, f_hi - 1, local, T, br)
+    const double b_lo = f_hi != 0 ? PartEvaluateAt(W, H, P, f_hi - 1, local, T, br)
                                   : -E_s / (kBoltzmann * T);
     const double b_hi =
-        f_hi != P.nF ? PartEvaluateAt(W, P, f_hi, local, T, br) : t.beta_cutoff;
+        f_hi != P.nF ? PartEvaluateAt(W, H, P, f_hi, local, T, br) : t.beta_cutoff;
     const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
     const double b_min = -E / (kBoltzmann * T);
     if (b_min <= b) {
@@ -479,17 +495,18 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const DTsl& t, uin
 // two bisections are independent and run in lockstep here so that their
 // memory round trips overlap.  Each follows exactly the probe sequence of a
 // single std::upper_bound.
-__device__ __forceinline__ void FindCdf2(const DevWorld& W, const DTslPart& P, int local,
-                                         double T, TBracket br, double a0, double a1,
+__device__ __forceinline__ void FindCdf2(const DevWorld& W, const double* H,
+                                         const DTslPart& P, int local, double T,
+                                         TBracket br, double a0, double a1,
                                          double alpha_cutoff, double& F0, double& F1) {
-  const double* F = W.pool + P.F_off;
+  const double* F = H + P.F_off;
   const int nF = P.nF;
   int first0 = 0, len0 = nF, first1 = 0, len1 = nF;
   while (len0 > 0 || len1 > 0) {
     const int half0 = len0 >> 1, half1 = len1 >> 1;
     const int mid0 = first0 + half0, mid1 = first1 + half1;
-    const double v0 = len0 > 0 ? PartEvaluateAt(W, P, mid0, local, T, br) : 0;
-    const double v1 = len1 > 0 ? PartEvaluateAt(W, P, mid1, local, T, br) : 0;
+    const double v0 = len0 > 0 ? PartEvaluateAt(W, H, P, mid0, local, T, br) : 0;
+    const double v1 = len1 > 0 ? PartEvaluateAt(W, H, P, mid1, local, T, br) : 0;
     if (len0 > 0) {
       if (a0 < v0) {
         len0 = half0;
@@ -508,30 +525,22 @@ __device__ __forceinline__ void FindCdf2(const DevWorld& W, const DTslPart& P, i
     }
   }
   const int h0 = first0, h1 = first1;
-  const double a_lo0 = h0 != 0 ? PartEvaluateAt(W, P, h0 - 1, local, T, br) : 0;
-  const double a_hi0 = h0 != nF ? PartEvaluateAt(W, P, h0, local, T, br) : alpha_cutoff;
-  const double a_lo1 = h1 != 0 ? PartEvaluateAt(W, P, h1 - 1, local, T, br) : 0;
-  const double a_hi1 = h1 != nF ? PartEvaluateAt(W, P, h1, local, T, br) : alpha_cutoff;
+  const double a_lo0 = h0 != 0 ? PartEvaluateAt(W, H, P, h0 - 1, local, T, br) : 0;
+  const double a_hi0 = h0 != nF ? PartEvaluateAt(W, H, P, h0, local, T, br) : alpha_cutoff;
+  const double a_lo1 = h1 != 0 ? PartEvaluateAt(W, H, P, h1 - 1, local, T, br) : 0;
+  const double a_hi1 = h1 != nF ? PartEvaluateAt(W, H, P, h1, local, T, br) : alpha_cutoff;
   const double Fl0 = h0 != 0 ? F[h0 - 1] : 0, Fh0 = h0 != nF ? F[h0] : 1;
   const double Fl1 = h1 != 0 ? F[h1 - 1] : 0, Fh1 = h1 != nF ? F[h1] : 1;
   F0 = Fl0 + (a0 - a_lo0) * (Fh0 - Fl0) / (a_hi0 - a_lo0);
   F1 = Fl1 + (a1 - a_lo1) * (Fh1 - Fl1) / (a_hi1 - a_lo1);
 }
-// Partition of a concatenated grid index (beta_partition_E_ends /
-// alpha_partition_beta_ends) and the index local to it.
-__device__ __forceinline__ int PartitionOf(const int* ends, int n_parts, int idx, int& local) {
-  int p_i = 0;
-  while (p_i < n_parts && ends[p_i] <= idx) p_i++;
-  local = p_i == 0 ? idx : idx - ends[p_i - 1];
-  return p_i;
-}
 
 // ThermalScattering::SampleAlpha (ThermalScattering.cpp:340-463), quirk Q2
 // (operator precedence at :361-365) reproduced.
-__device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t,
-                                                uint32_t& g, double b, double E,
-                                                double T, double& alpha) {
-  const double* betas = W.pool + t.betas_off;
+__device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const double* H,
+                                                const DTsl& t, uint32_t& g, double b,
+                                                double E, double T, double& alpha) {
+  const double* betas = H + t.betas_off;
   const double abs_b = fabs(b);
   const int sgn = (0 < b) - (b < 0);
   const int b_hi = UpperBoundArr(betas, t.nbetas, abs_b);
@@ -556,23 +565,21 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const DTsl& t
   const double dm = sqrt_E - bs_root, dp = sqrt_E + bs_root;
   const double a_min = dm * dm / akT;
   const double a_max = dp * dp / akT;
-  const int* ends = W.ipool + t.beta_ends_off;
-  int p_i = 0;
-  while (p_i < t.n_ap && ends[p_i] <= b_s_i) p_i++;
+  int local;
+  const int p_i = PartitionOf(H + t.beta_ends_off, t.n_ap, b_s_i, local);
   const DTslPart P = W.parts[t.ap_begin + p_i];
-  const int local = p_i == 0 ? b_s_i : b_s_i - ends[p_i - 1];
-  const double* F = W.pool + P.F_off;
+  const double* F = H + P.F_off;
   const int nF = P.nF;
-  const TBracket br = PartTBracket(W, P, T);
+  const TBracket br = PartTBracket(H, P, T);
   double F_min, F_max;
-  FindCdf2(W, P, local, T, br, a_min, a_max, t.alpha_cutoff, F_min, F_max);
+  FindCdf2(W, H, P, local, T, br, a_min, a_max, t.alpha_cutoff, F_min, F_max);
   for (int k = 0; k < kAlphaResampleLimit; k++) {
     const double u = F_min + Canonical(g) * (F_max - F_min);
     const int hi = UpperBoundArr(F, nF, u);
     const double F_lo = hi != 0 ? F[hi - 1] : 0;
     const double F_hi = hi != nF ? F[hi] : 1;
-    const double a_lo = hi != 0 ? PartEvaluateAt(W, P, hi - 1, local, T, br) : 0;
-    const double a_hi = hi != nF ? PartEvaluateAt(W, P, hi, local, T, br) : t.alpha_cutoff;
+    const double a_lo = hi != 0 ? PartEvaluateAt(W, H, P, hi - 1, local, T, br) : 0;
+    const double a_hi = hi != nF ? PartEvaluateAt(W, H, P, hi, local, T, br) : t.alpha_cutoff;
     const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
     if (a_min < a && a < a_max) {
       const double b_root = sqrt(E + b * kBoltzmann * T);

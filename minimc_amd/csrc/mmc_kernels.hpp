@@ -695,12 +695,14 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld W, RunState R, int i
 #endif  // MMC_TRACKER_KERNELS
 
 // ThermalScattering::Scatter (ThermalScattering.cpp:159-171) with POD sampling.
-__device__ __forceinline__ uint32_t TslScatterPod(const DevWorld& W, const DTsl& t,
-                                                  uint32_t& g, double T, V3& dir,
-                                                  double& E, double& alpha, double& beta) {
+// H: the sampler's hot block (DevWorld::hot or its LDS copy).
+__device__ __forceinline__ uint32_t TslScatterPod(const DevWorld& W, const double* H,
+                                                  const DTsl& t, uint32_t& g, double T,
+                                                  V3& dir, double& E, double& alpha,
+                                                  double& beta) {
   const double E0 = E;
-  if (!SampleBeta(W, t, g, E0, T, beta)) return kErrResample;
-  const uint32_t e = SampleAlpha(W, t, g, beta, E0, T, alpha);
+  if (!SampleBeta(W, H, t, g, E0, T, beta)) return kErrResample;
+  const uint32_t e = SampleAlpha(W, H, t, g, beta, E0, T, alpha);
   if (e) return e;
   const double E_p = E0 + beta * kBoltzmann * T;
   const double mu = (E0 + E_p - alpha * t.awr * kBoltzmann * T) / (2 * sqrt(E0 * E_p));
@@ -712,12 +714,13 @@ __device__ __forceinline__ uint32_t TslScatterPod(const DevWorld& W, const DTsl&
 
 // Thermal scattering of particle p with the POD sampler; false on a
 // sampling error (flagged in misc).
-__device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R, uint32_t p) {
+__device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R, uint32_t p,
+                                            const double* H) {
   V3 dir = LoadDir(R.B, p);
   double E = R.B.E[p];
   uint32_t g = R.B.rng[p];
   double a = 0, b = 0;
-  const uint32_t e = TslScatterPod(W, W.tsl[R.B.tslI[p]], g, R.B.tslT[p], dir, E, a, b);
+  const uint32_t e = TslScatterPod(W, H, W.tsl[R.B.tslI[p]], g, R.B.tslT[p], dir, E, a, b);
   if (e) {
     atomicOr(R.misc, e);
     return false;
@@ -729,9 +732,36 @@ __device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R
   return true;
 }
 
+// Copies the sampler's hot block into LDS (block-wide; every thread of the
+// block must call it).
+__device__ __forceinline__ const double* StageHot(const DevWorld& W, double* lds) {
+  constexpr int kU = 8;
+  for (int k0 = threadIdx.x; k0 < W.hot_n; k0 += kU * blockDim.x) {
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      v[u] = k < W.hot_n ? W.hot[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      if (k < W.hot_n) lds[k] = v[u];
+    }
+  }
+  __syncthreads();
+  return lds;
+}
+
 #ifdef MMC_TRACKER_KERNELS
+// kLds: the hot block is staged in LDS (launched with hot_n doubles of
+// dynamic LDS and only as many blocks as are resident at once).
+template <bool kLds>
 __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
+  extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter, kT);
+  if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
+  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -739,7 +769,7 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
     uint32_t p = 0;
     if (valid) {
       p = R.QT[q];
-      alive = TslPodEvent(W, R, p);
+      alive = TslPodEvent(W, R, p, H);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
@@ -801,9 +831,13 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 // history ends takes the next queued particle.  Thermal scatterings use the
 // POD sampler here.  Used only where a collision cannot start a new particle
 // (k-eigenvalue, or no fission).
+template <bool kLds>
 __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCrs = 1, kCol = 2, kTsl = 3, kDone = 4 };
+  extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter - 1, kQ);
+  if (blockIdx.x * blockDim.x >= n) return;
+  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   int st = q < n ? kAdv : kDone;
@@ -837,7 +871,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, i
       st = to_t ? kTsl : (alive ? kAdv : kDone);
     } else {
       c_tsl++;
-      st = TslPodEvent(W, R, p) ? kAdv : kDone;
+      st = TslPodEvent(W, R, p, H) ? kAdv : kDone;
     }
     if (st == kDone) {
       q += stride;

@@ -380,18 +380,18 @@ struct PodRow {
   double c[16];
 };
 __device__ void MakeRow(const DevWorld& W, const DTslPart& P, int x_i, double T, PodRow& row) {
-  const double* Ts = W.pool + P.T_off;
+  const double* Ts = W.hot + P.T_off;
   const int cand = UpperBoundArr(Ts, P.nT, T);
   const int t_hi = cand == P.nT ? cand - 1 : cand;
   const int t_lo = t_hi == 0 ? t_hi : t_hi - 1;
   const double w = t_hi == t_lo ? 0.0 : (T - Ts[t_lo]) / (Ts[t_hi] - Ts[t_lo]);
-  const double* S = W.pool + P.S_off;
+  const double* S = W.hot + P.S_off;
   const double* vh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
   const double* vl = W.pool + P.xt_off + (x_i * P.nT + t_lo) * P.rank;
   row.rank = P.rank < 16 ? P.rank : 16;
   for (int r = 0; r < row.rank; r++) row.c[r] = S[r] * (vl[r] + w * (vh[r] - vl[r]));
-  row.cdf = W.pool + P.cdf_off;
-  row.F = W.pool + P.F_off;
+  row.cdf = W.hot + P.cdf_off;
+  row.F = W.hot + P.F_off;
   row.nF = P.nF;
 }
 __device__ __forceinline__ double RowValue(const PodRow& row, int j) {
@@ -453,7 +453,7 @@ __device__ double RowMeasure(const PodRow& row, double lo_end, double hi_end, do
 }
 
 __device__ double PodMapBeta(const DevWorld& W, const DTsl& t, double E, double T, double u) {
-  const double* Es = W.pool + t.Es_off;
+  const double* Es = W.hot + t.Es_off;
   int e_hi = UpperBoundArr(Es, t.nEs, E);
   if (e_hi >= t.nEs) e_hi = t.nEs - 1;
   int g[2];
@@ -473,7 +473,7 @@ __device__ double PodMapBeta(const DevWorld& W, const DTsl& t, double E, double 
   double lo_end[2], c0[2];
   for (int q = 0; q < ng; q++) {
     int local;
-    const int p_i = PartitionOf(W.ipool + t.E_ends_off, t.n_bp, g[q], local);
+    const int p_i = PartitionOf(W.hot + t.E_ends_off, t.n_bp, g[q], local);
     MakeRow(W, W.parts[t.bp_begin + p_i], local, T, rows[q]);
     lo_end[q] = -Es[g[q]] / kT;
     c0[q] = RowMeasure(rows[q], lo_end[q], t.beta_cutoff, b_min, 0.0, 1.0);  // rejected
@@ -497,7 +497,7 @@ __device__ double PodMapBeta(const DevWorld& W, const DTsl& t, double E, double 
 
 __device__ double PodMapAlphaFrac(const DevWorld& W, const DTsl& t, double b, double E,
                                   double T, double u) {
-  const double* betas = W.pool + t.betas_off;
+  const double* betas = W.hot + t.betas_off;
   const double kT = kBoltzmann * T;
   const double ab = fabs(b);
   const int sgn = (0 < b) - (b < 0);
@@ -532,7 +532,7 @@ __device__ double PodMapAlphaFrac(const DevWorld& W, const DTsl& t, double b, do
     amin[q] = (sqrt_E - root) * (sqrt_E - root) / akT;
     amax[q] = (sqrt_E + root) * (sqrt_E + root) / akT;
     int local;
-    const int p_i = PartitionOf(W.ipool + t.beta_ends_off, t.n_ap, g[q], local);
+    const int p_i = PartitionOf(W.hot + t.beta_ends_off, t.n_ap, g[q], local);
     MakeRow(W, W.parts[t.ap_begin + p_i], local, T, rows[q]);
     // find_cdf of the reference (monotone search) fixes the F window ...
     Fmin[q] = RowCdf(rows[q], 0.0, t.alpha_cutoff, amin[q]);
@@ -598,7 +598,7 @@ __global__ void __launch_bounds__(256) KTslPodFallback(DevWorld W, RunState R, i
     uint32_t p = 0;
     if (q < n) {
       p = fb[q];
-      alive = TslPodEvent(W, R, p);
+      alive = TslPodEvent(W, R, p, W.hot);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter - 1, kQ), alive, tot);
@@ -1109,8 +1109,8 @@ __global__ void KSurrogateFinish(DevWorld W, MlpState m, int tsl, uint32_t n, co
       !OutputsToAlphaBeta(yout[size_t(i) * 2], yout[size_t(i) * 2 + 1], E[i], T[i], t.awr,
                           t.beta_cutoff, m.e_min, a, b)) {
     uint32_t g = rng[i];
-    uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
-    if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);
+    uint32_t e = SampleBeta(W, W.hot, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
+    if (!e) e = SampleAlpha(W, W.hot, t, g, b, E[i], T[i], a);
     if (e) {
       atomicOr(err, e);
       a = b = __builtin_nan("");

@@ -28,6 +28,8 @@ using namespace mmc;
   } while (0)
 
 static constexpr int kBlock = 256;
+// Largest thermal-scattering hot block staged in LDS (of 160 KiB per CU).
+static constexpr size_t kHotLdsMax = 128 * 1024;
 
 struct mmc_ctx {
   std::string err;
@@ -41,6 +43,8 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
+  size_t hot_lds = 0;          // LDS bytes of the sampler's hot block (0: not staged)
+  uint32_t tsl_grid = 1024;    // grid cap of KTslPod when it stages the hot block
   DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
@@ -104,6 +108,7 @@ namespace {
 
 struct Flat {
   std::vector<double> pool;
+  std::vector<double> hot;  // the thermal-scattering sampler's tables (DevWorld::hot)
   std::vector<int> ipool;
   std::vector<DGrid> grids;
   std::vector<std::vector<double>> grid_keys;
@@ -112,6 +117,11 @@ struct Flat {
   int AddDoubles(const double* p, size_t n) {
     const int off = int(pool.size());
     pool.insert(pool.end(), p, p + n);
+    return off;
+  }
+  int AddHot(const double* p, size_t n) {
+    const int off = int(hot.size());
+    hot.insert(hot.end(), p, p + n);
     return off;
   }
   int AddGrid(const double* x, int n) {
@@ -367,13 +377,13 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
         std::vector<int> ends;
         for (int k = 0; k < np; k++) {
           DTslPart q{};
-          q.F_off = F.AddDoubles(p[k].F, p[k].n_F);
+          q.F_off = F.AddHot(p[k].F, p[k].n_F);
           q.nF = p[k].n_F;
-          q.cdf_off = F.AddDoubles(p[k].cdf_modes, size_t(p[k].n_F) * p[k].rank);
-          q.S_off = F.AddDoubles(p[k].S, p[k].rank);
+          q.cdf_off = F.AddHot(p[k].cdf_modes, size_t(p[k].n_F) * p[k].rank);
+          q.S_off = F.AddHot(p[k].S, p[k].rank);
           q.x_off = F.AddDoubles(p[k].x, p[k].n_x);
           q.nx = p[k].n_x;
-          q.T_off = F.AddDoubles(p[k].T, p[k].n_T);
+          q.T_off = F.AddHot(p[k].T, p[k].n_T);
           q.nT = p[k].n_T;
           q.xt_off = F.AddDoubles(p[k].x_T_modes, size_t(p[k].n_x) * p[k].n_T * p[k].rank);
           q.rank = p[k].rank;
@@ -381,10 +391,10 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
           xs.insert(xs.end(), p[k].x, p[k].x + p[k].n_x);
           ends.push_back(int(xs.size()));
         }
-        xs_off = F.AddDoubles(xs.data(), xs.size());
+        xs_off = F.AddHot(xs.data(), xs.size());
         nxs = int(xs.size());
-        ends_off = int(F.ipool.size());
-        F.ipool.insert(F.ipool.end(), ends.begin(), ends.end());
+        const std::vector<double> ends_d(ends.begin(), ends.end());
+        ends_off = F.AddHot(ends_d.data(), ends_d.size());
       };
       add_parts(d.beta_partitions, d.n_beta_partitions, t.bp_begin, t.Es_off, t.nEs,
                 t.E_ends_off);
@@ -473,6 +483,12 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   W.parts = DevUpload(ctx, parts);
   W.pool = DevUpload(ctx, F.pool);
   W.ipool = DevUpload(ctx, F.ipool);
+  W.hot_n = int(F.hot.size());
+  if (F.hot.empty()) F.hot.push_back(0);
+  W.hot = DevUpload(ctx, F.hot);
+  // the sampling kernels copy the hot block into LDS when it fits
+  ctx->hot_lds = W.hot_n > 0 && size_t(W.hot_n) * sizeof(double) <= kHotLdsMax
+                     ? size_t(W.hot_n) * sizeof(double) : 0;
   W.ests = DevUpload(ctx, ests);
   ctx->S = DevAlloc<double>(ctx, nb);
   ctx->S2 = DevAlloc<double>(ctx, nb);
@@ -607,9 +623,19 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     // the tail kernel's lanes take the next particle when one ends: launch
     // only the blocks that are resident at once
     int occ_t = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail, kBlock, 0) != hipSuccess)
-      occ_t = 1;
+    const hipError_t et =
+        ctx->hot_lds
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<true>, kBlock,
+                                                           ctx->hot_lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<false>, kBlock, 0);
+    if (et != hipSuccess) occ_t = 1;
     ctx->tail_grid = uint32_t(std::max(1, occ_t)) * uint32_t(n_cu);
+    // KTslPod with the hot block in LDS: resident blocks only
+    int occ_s = 0;
+    if (ctx->hot_lds && hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &occ_s, KTslPod<true>, kBlock, ctx->hot_lds) != hipSuccess)
+      occ_s = 1;
+    ctx->tsl_grid = uint32_t(std::max(1, occ_s)) * uint32_t(n_cu);
   }
   uint64_t cap = ctx->opt.max_in_flight ? ctx->opt.max_in_flight : (1ull << 20);
   cap = std::min<uint64_t>(std::max<uint64_t>(cap, 1024), 1ull << 28);
@@ -815,8 +841,12 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
     } else {
       ProfBegin(ctx, MMC_K_TSL);
-      hipLaunchKernelGGL(KTslPod, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
-                         iter, qin);
+      if (ctx->hot_lds)
+        hipLaunchKernelGGL(KTslPod<true>, dim3(std::min(grid, ctx->tsl_grid)), dim3(kBlock),
+                           ctx->hot_lds, ctx->stream, ctx->W, ctx->R, iter, qin);
+      else
+        hipLaunchKernelGGL(KTslPod<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W,
+                           ctx->R, iter, qin);
       ProfEnd(ctx);
     }
   }
@@ -895,8 +925,13 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
       ProfBegin(ctx, MMC_K_TAIL);
-      hipLaunchKernelGGL(KTail, dim3(std::min(Blocks(bound), ctx->tail_grid)), dim3(kBlock), 0,
-                         ctx->stream, ctx->W, ctx->R, iter, qin);
+      const dim3 tgrid(std::min(Blocks(bound), ctx->tail_grid));
+      if (ctx->hot_lds)
+        hipLaunchKernelGGL(KTail<true>, tgrid, dim3(kBlock), ctx->hot_lds, ctx->stream, ctx->W,
+                           ctx->R, iter, qin);
+      else
+        hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
+                           iter, qin);
       ProfEnd(ctx);
       HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
                             ctx->stream));
@@ -1360,7 +1395,7 @@ __global__ void KEval(DevWorld W, int kind, EvalSoA s, uint32_t n, uint32_t* err
       if (o.tsl) {
         double a, b;
         const double T = FieldAt(W.fields[W.cells[cell].field], pos);
-        const uint32_t e = TslScatterPod(W, W.tsl[tsl], g, T, dir, E, a, b);
+        const uint32_t e = TslScatterPod(W, W.hot, W.tsl[tsl], g, T, dir, E, a, b);
         if (e) atomicOr(err, e);
       }
       if (o.event >= 0) s.event[i] = o.event;
@@ -1377,8 +1412,9 @@ __global__ void KEval(DevWorld W, int kind, EvalSoA s, uint32_t n, uint32_t* err
     case MMC_EVAL_TSL_SAMPLE: {
       double a = 0, b = 0;
       const DTsl t = W.tsl[0];
-      uint32_t e = SampleBeta(W, t, g, E, s.out0[i], b) ? 0u : uint32_t(kErrResample);
-      if (!e) e = SampleAlpha(W, t, g, b, E, s.out0[i], a);
+      uint32_t e =
+          SampleBeta(W, W.hot, t, g, E, s.out0[i], b) ? 0u : uint32_t(kErrResample);
+      if (!e) e = SampleAlpha(W, W.hot, t, g, b, E, s.out0[i], a);
       if (e) {
         atomicOr(err, e);
         a = b = __builtin_nan("");
@@ -1513,22 +1549,28 @@ extern "C" int mmc_sample_source(mmc_ctx* ctx, const mmc_source_desc* src, uint6
 
 // ---------------------------------------------------------------------------
 // Batch POD sampling of (alpha, beta) on device-resident inputs.
-__global__ void KTslBatch(DevWorld W, int tsl, uint32_t n, const double* E, const double* T,
-                          uint32_t* rng, double* alpha, double* beta, uint32_t* err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t g = rng[i];
-  double a = 0, b = 0;
+template <bool kLds>
+__global__ void __launch_bounds__(256)
+    KTslBatch(DevWorld W, int tsl, uint32_t n, const double* E, const double* T, uint32_t* rng,
+              double* alpha, double* beta, uint32_t* err) {
+  extern __shared__ double lds_hot[];
+  if (blockIdx.x * blockDim.x >= n) return;
+  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
   const DTsl t = W.tsl[tsl];
-  uint32_t e = SampleBeta(W, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
-  if (!e) e = SampleAlpha(W, t, g, b, E[i], T[i], a);
-  if (e) {
-    atomicOr(err, e);
-    a = b = __builtin_nan("");
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    uint32_t g = rng[i];
+    double a = 0, b = 0;
+    uint32_t e = SampleBeta(W, H, t, g, E[i], T[i], b) ? 0u : uint32_t(kErrResample);
+    if (!e) e = SampleAlpha(W, H, t, g, b, E[i], T[i], a);
+    if (e) {
+      atomicOr(err, e);
+      a = b = __builtin_nan("");
+    }
+    alpha[i] = a;
+    beta[i] = b;
+    rng[i] = g;
   }
-  alpha[i] = a;
-  beta[i] = b;
-  rng[i] = g;
 }
 
 extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
@@ -1540,8 +1582,14 @@ extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const doubl
   if (n > 0xffffffffull) return Fail(ctx, MMC_ERR_ARG, "batch too large");
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipMemsetAsync(ctx->R.misc, 0, sizeof(uint32_t), ctx->stream));
-  hipLaunchKernelGGL(KTslBatch, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
-                     ctx->W, tsl, uint32_t(n), E, T, rng, alpha, beta, ctx->R.misc);
+  if (ctx->hot_lds)
+    hipLaunchKernelGGL(KTslBatch<true>, dim3(std::min(Blocks(n, 1u << 30), ctx->tsl_grid)),
+                       dim3(kBlock), ctx->hot_lds, ctx->stream, ctx->W, tsl, uint32_t(n), E, T,
+                       rng, alpha, beta, ctx->R.misc);
+  else
+    hipLaunchKernelGGL(KTslBatch<false>, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0,
+                       ctx->stream, ctx->W, tsl, uint32_t(n), E, T, rng, alpha, beta,
+                       ctx->R.misc);
   uint32_t misc[16] = {0};
   HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(uint32_t), hipMemcpyDeviceToHost,
                         ctx->stream));
