@@ -82,10 +82,12 @@ struct DCeNuclide {
   int grid;              // the grid shared by every table of this nuclide, or -1
   int fissile;
 };
-// The sampler's axes and CDF mode tables (F_off, cdf_off, S_off, T_off of a
-// partition; Es_off, E_ends_off, betas_off, beta_ends_off of a table) are
-// offsets into DevWorld::hot, a block small enough to be copied into LDS by
-// the kernels that sample.  x_off and xt_off are offsets into the pool.
+// Two small blocks that kernels copy into LDS: DevWorld::tot holds every
+// table's total cross-section tables (E_axis_off, E_modes_off, T_axis_off,
+// T_modes_off, S_off of DTsl), DevWorld::hot the sampler's axes and CDF mode
+// tables (F_off, cdf_off, S_off, T_off of a partition; Es_off, E_ends_off,
+// betas_off, beta_ends_off of a table).  x_off and xt_off are offsets into
+// the pool.
 struct DTslPart {
   int F_off, nF, cdf_off, S_off, x_off, nx, T_off, nT, xt_off, rank;
 };
@@ -135,7 +137,8 @@ struct DevWorld {
   const int* ipool;
   const DEst* ests;
   const double* hot;  // thermal-scattering sampler tables (see DTsl)
-  int hot_n, pad_h;   // doubles in hot
+  const double* tot;  // thermal-scattering total cross-section tables
+  int hot_n, tot_n;   // doubles in each
 };
 
 // ---- RNG: std::minstd_rand with libstdc++ canonical (see oracle/rng.hpp) --
@@ -414,10 +417,10 @@ __device__ __forceinline__ double PartEvaluate(const DevWorld& W, const double* 
   return PartEvaluateAt(W, H, P, cdf_i, x_i, T, PartTBracket(H, P, T));
 }
 // ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
-__device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, double E,
+__device__ __forceinline__ double TslTotal(const double* H, const DTsl& t, double E,
                                            double T) {
-  const double* Ex = W.pool + t.E_axis_off;
-  const double* Tx = W.pool + t.T_axis_off;
+  const double* Ex = H + t.E_axis_off;
+  const double* Tx = H + t.T_axis_off;
   const int e_hi = UpperBoundArr(Ex, t.nE, E);
   const bool below_E = e_hi == 0;
   const int e_lo = below_E ? e_hi : e_hi - 1;
@@ -426,9 +429,9 @@ __device__ __forceinline__ double TslTotal(const DevWorld& W, const DTsl& t, dou
   const int t_hi = above_T ? cand - 1 : cand;
   const bool below_T = t_hi == 0;
   const int t_lo = below_T ? t_hi : t_hi - 1;
-  const double* S = W.pool + t.S_off;
-  const double* Em = W.pool + t.E_modes_off;
-  const double* Tm = W.pool + t.T_modes_off;
+  const double* S = H + t.S_off;
+  const double* Em = H + t.E_modes_off;
+  const double* Tm = H + t.T_modes_off;
   double ll = 0, lh = 0, hl = 0, hh = 0;
   for (int o = 0; o < t.rank; o++) {
     const double sel = S[o] * Em[e_lo * t.rank + o];
@@ -620,9 +623,11 @@ struct XsMemo {
   int hint_grid = -1, hint_gi = 0;
   int tsl = -1;
   double tE = 0, tT = 0, tval = 0;
+  const double* tot = nullptr;  // DevWorld::tot or its copy in LDS (never null in use)
 };
 
 // Grid index of E for a nuclide whose tables share one grid, or -1.
+// lds_hash: grid 0's bucket hash staged in LDS, or nullptr.
 __device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNuclide& n,
                                                 const int* lds_hash, double E,
                                                 XsMemo* memo = nullptr) {
@@ -635,7 +640,9 @@ __device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNucl
     const int i = memo->hint_gi;
     if (i >= 0 && i <= g.n && (i == 0 || x[i - 1] <= E) && (i == g.n || x[i] > E)) gi = i;
   }
-  if (gi < 0) gi = GridUpperBound(W, g, lds_hash ? lds_hash : W.ipool + g.hash_off, E);
+  // lds_hash is the LDS copy of grid 0's hash; other grids use their own
+  if (gi < 0)
+    gi = GridUpperBound(W, g, lds_hash && n.grid == 0 ? lds_hash : W.ipool + g.hash_off, E);
   if (memo) {
     memo->grid = n.grid;
     memo->E = E;
@@ -656,7 +663,7 @@ __device__ __forceinline__ double RxXs(const DevWorld& W, const DCeNuclide& n,
     if (E < t.cutoff_energy) {
       if (majorant) return TableAt(W, t.majorant, E);
       if (memo && memo->tsl == r.tsl && memo->tE == E && memo->tT == T) return memo->tval;
-      const double v = TslTotal(W, t, E, T);
+      const double v = TslTotal(memo ? memo->tot : W.tot, t, E, T);
       if (memo) {
         memo->tsl = r.tsl;
         memo->tE = E;

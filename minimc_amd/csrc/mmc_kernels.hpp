@@ -231,6 +231,35 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
 
 #endif  // MMC_TRACKER_KERNELS
 
+// LDS layout of the kernels that stage tables: the first grid's bucket hash
+// (n_hash + 1 ints, padded to an even count) when there is one, then the
+// thermal-scattering totals (tot_n doubles) and/or the sampler's tables
+// (hot_n doubles), as the kernel needs them.
+__device__ __forceinline__ int HashLdsInts(const DevWorld& W) {
+  return (W.energy_mode == 1 && W.grids[0].n_hash > 0) ? ((W.grids[0].n_hash + 2) & ~1) : 0;
+}
+
+// Copies src[0, n) into LDS (block-wide; every thread of the block must call
+// it).
+__device__ __forceinline__ const double* StageHot(const double* src, double* lds, int n) {
+  constexpr int kU = 8;
+  for (int k0 = threadIdx.x; k0 < n; k0 += kU * blockDim.x) {
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      v[u] = k < n ? src[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * blockDim.x;
+      if (k < n) lds[k] = v[u];
+    }
+  }
+  __syncthreads();
+  return lds;
+}
+
 // Copies the first grid's bucket hash into LDS (block-wide; every thread of
 // the block must call it).  Returns the pointer lookups use, or nullptr when
 // there is no hash to stage.
@@ -265,8 +294,11 @@ __device__ __forceinline__ const int* StageHash(const DevWorld& W, int* lds_hash
 struct AdvanceOut {
   bool to_x, to_c, to_q;
 };
+// hash: the energy hash (LDS) or nullptr; Ht: the thermal-scattering totals
+// (DevWorld::tot or its LDS copy).
 __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
-                                                   uint32_t p, const int* hash) {
+                                                   uint32_t p, const int* hash,
+                                                   const double* Ht) {
   AdvanceOut o{false, false, false};
   const bool delta = W.tracking == 1;
   V3 pos = LoadPos(R.B, p);
@@ -281,6 +313,7 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   XsMemo memo;
   memo.hint_grid = R.B.xs_grid[p];
   memo.hint_gi = R.B.xs_gi[p];
+  memo.tot = Ht;
   const double sigma = MaterialTotal(W, c.material, E, T, delta, hash, &memo);
   const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
   int s;
@@ -321,12 +354,17 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
 }
 
 #ifdef MMC_TRACKER_KERNELS
+// kLds: the thermal-scattering totals are staged after the hash.
+template <bool kLds>
 __global__ void __launch_bounds__(256) KAdvance(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
   if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
   const uint32_t n = *Ring(R, iter - 1, kQ);
   if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
   const int* hash = StageHash(W, lds_hash);
+  const double* Ht =
+      kLds ? StageHot(W.tot, reinterpret_cast<double*>(lds_hash + HashLdsInts(W)), W.tot_n)
+           : W.tot;
   const uint32_t* Q = R.Q[qin];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
@@ -335,7 +373,7 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld W, RunState R, int iter
     uint32_t p = 0;
     if (valid) {
       p = Q[q];
-      o = AdvanceEvent(W, R, p, hash);
+      o = AdvanceEvent(W, R, p, hash, Ht);
     }
     uint32_t tot;
     const uint32_t ix = WavePush(Ring(R, iter, kX), o.to_x, tot);
@@ -629,8 +667,9 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
 // thermal scattering whose (alpha, beta) is sampled by the TSL step.
 // Children of a fixed-source fission go to live_q / live_cnt.
 __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& R, uint32_t p,
-                                             const int* hash, uint32_t* live_q,
-                                             uint32_t* live_cnt, bool& alive, bool& to_t) {
+                                             const int* hash, const double* Ht,
+                                             uint32_t* live_q, uint32_t* live_cnt, bool& alive,
+                                             bool& to_t) {
   alive = to_t = false;
   const V3 pos = LoadPos(R.B, p);
   V3 dir = LoadDir(R.B, p);
@@ -642,6 +681,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   XsMemo memo;
   memo.hint_grid = R.B.xs_grid[p];
   memo.hint_gi = R.B.xs_gi[p];
+  memo.tot = Ht;
   if (W.n_est_flux || R.keig) {
     const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo);
     if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
@@ -669,11 +709,15 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
 }
 
 #ifdef MMC_TRACKER_KERNELS
+template <bool kLds>
 __global__ void __launch_bounds__(256, 2) KCollide(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter, kC);
   if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
   const int* hash = StageHash(W, lds_hash);
+  const double* Ht =
+      kLds ? StageHot(W.tot, reinterpret_cast<double*>(lds_hash + HashLdsInts(W)), W.tot_n)
+           : W.tot;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -681,7 +725,7 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld W, RunState R, int i
     uint32_t p = 0;
     if (valid) {
       p = R.QC[q];
-      CollideEvent(W, R, p, hash, R.Q[qin ^ 1], Ring(R, iter, kQ), alive, to_t);
+      CollideEvent(W, R, p, hash, Ht, R.Q[qin ^ 1], Ring(R, iter, kQ), alive, to_t);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
@@ -732,27 +776,6 @@ __device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R
   return true;
 }
 
-// Copies the sampler's hot block into LDS (block-wide; every thread of the
-// block must call it).
-__device__ __forceinline__ const double* StageHot(const DevWorld& W, double* lds) {
-  constexpr int kU = 8;
-  for (int k0 = threadIdx.x; k0 < W.hot_n; k0 += kU * blockDim.x) {
-    double v[kU];
-#pragma unroll
-    for (int u = 0; u < kU; u++) {
-      const int k = k0 + u * blockDim.x;
-      v[u] = k < W.hot_n ? W.hot[k] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kU; u++) {
-      const int k = k0 + u * blockDim.x;
-      if (k < W.hot_n) lds[k] = v[u];
-    }
-  }
-  __syncthreads();
-  return lds;
-}
-
 #ifdef MMC_TRACKER_KERNELS
 // kLds: the hot block is staged in LDS (launched with hot_n doubles of
 // dynamic LDS and only as many blocks as are resident at once).
@@ -761,7 +784,7 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
   extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter, kT);
   if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
-  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
+  const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -834,10 +857,14 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 template <bool kLds>
 __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCrs = 1, kCol = 2, kTsl = 3, kDone = 4 };
-  extern __shared__ double lds_hot[];
+  extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter - 1, kQ);
   if (blockIdx.x * blockDim.x >= n) return;
-  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
+  // kLds: the hash, the totals and the sampler's tables in LDS
+  const int* hash = kLds ? StageHash(W, lds_hash) : nullptr;
+  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(W));
+  const double* Ht = kLds ? StageHot(W.tot, lds_tot, W.tot_n) : W.tot;
+  const double* H = kLds ? StageHot(W.hot, lds_tot + W.tot_n, W.hot_n) : W.hot;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   int st = q < n ? kAdv : kDone;
@@ -857,7 +884,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, i
     if (pick < 0) break;  // every lane of the wave is done
     if (st != pick) continue;
     if (st == kAdv) {
-      const AdvanceOut o = AdvanceEvent(W, R, p, nullptr);
+      const AdvanceOut o = AdvanceEvent(W, R, p, hash, Ht);
       c_adv++;
       c_virt += o.to_q ? 1 : 0;
       st = o.to_q ? kAdv : (o.to_x ? kCrs : (o.to_c ? kCol : kDone));
@@ -867,7 +894,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, i
     } else if (st == kCol) {
       c_coll++;
       bool alive, to_t;
-      CollideEvent(W, R, p, nullptr, nullptr, nullptr, alive, to_t);
+      CollideEvent(W, R, p, hash, Ht, nullptr, nullptr, alive, to_t);
       st = to_t ? kTsl : (alive ? kAdv : kDone);
     } else {
       c_tsl++;

@@ -28,8 +28,11 @@ using namespace mmc;
   } while (0)
 
 static constexpr int kBlock = 256;
-// Largest thermal-scattering hot block staged in LDS (of 160 KiB per CU).
+// Largest thermal-scattering hot block staged in LDS (of 160 KiB per CU),
+// and the largest total-cross-section part staged by the event kernels.
 static constexpr size_t kHotLdsMax = 128 * 1024;
+static constexpr size_t kTotLdsMax = 32 * 1024;
+static constexpr size_t kLdsPerBlock = 160 * 1024;
 
 struct mmc_ctx {
   std::string err;
@@ -43,8 +46,11 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
-  size_t hot_lds = 0;          // LDS bytes of the sampler's hot block (0: not staged)
+  size_t hot_lds = 0;          // LDS bytes of the hot block (0: KTslPod does not stage it)
   uint32_t tsl_grid = 1024;    // grid cap of KTslPod when it stages the hot block
+  size_t ev_lds = 0;           // LDS bytes of KAdvance / KCollide
+  bool ev_tot_lds = false;     // ... which stage the hot block's totals after the hash
+  size_t tail_lds = 0;         // LDS bytes of KTail (0: KTail<false>)
   DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
@@ -109,6 +115,7 @@ namespace {
 struct Flat {
   std::vector<double> pool;
   std::vector<double> hot;  // the thermal-scattering sampler's tables (DevWorld::hot)
+  std::vector<double> tot;  // the thermal-scattering totals (DevWorld::tot)
   std::vector<int> ipool;
   std::vector<DGrid> grids;
   std::vector<std::vector<double>> grid_keys;
@@ -122,6 +129,11 @@ struct Flat {
   int AddHot(const double* p, size_t n) {
     const int off = int(hot.size());
     hot.insert(hot.end(), p, p + n);
+    return off;
+  }
+  int AddTot(const double* p, size_t n) {
+    const int off = int(tot.size());
+    tot.insert(tot.end(), p, p + n);
     return off;
   }
   int AddGrid(const double* x, int n) {
@@ -362,13 +374,13 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       t.majorant = F.AddTable(d.majorant);
       if (t.majorant < 0 || d.n_E < 1 || d.n_T < 1 || d.rank < 1)
         return Fail(ctx, MMC_ERR_ARG, "bad tsl tables");
-      t.E_axis_off = F.AddDoubles(d.total_E_axis, d.n_E);
+      t.E_axis_off = F.AddTot(d.total_E_axis, d.n_E);
       t.nE = d.n_E;
-      t.E_modes_off = F.AddDoubles(d.total_E, size_t(d.n_E) * d.rank);
-      t.T_axis_off = F.AddDoubles(d.total_T_axis, d.n_T);
+      t.E_modes_off = F.AddTot(d.total_E, size_t(d.n_E) * d.rank);
+      t.T_axis_off = F.AddTot(d.total_T_axis, d.n_T);
       t.nT = d.n_T;
-      t.T_modes_off = F.AddDoubles(d.total_T, size_t(d.n_T) * d.rank);
-      t.S_off = F.AddDoubles(d.total_S, d.rank);
+      t.T_modes_off = F.AddTot(d.total_T, size_t(d.n_T) * d.rank);
+      t.S_off = F.AddTot(d.total_S, d.rank);
       t.rank = d.rank;
       auto add_parts = [&](const mmc_tsl_partition* p, int np, int& begin, int& xs_off,
                            int& nxs, int& ends_off) {
@@ -486,9 +498,22 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   W.hot_n = int(F.hot.size());
   if (F.hot.empty()) F.hot.push_back(0);
   W.hot = DevUpload(ctx, F.hot);
-  // the sampling kernels copy the hot block into LDS when it fits
+  W.tot_n = int(F.tot.size());
+  if (F.tot.empty()) F.tot.push_back(0);
+  W.tot = DevUpload(ctx, F.tot);
+  // LDS use (layout as HashLdsInts on the device): KTslPod stages the
+  // sampler's tables; KAdvance and KCollide the energy hash, then the
+  // thermal-scattering totals; KTail the hash, the totals, then the sampler's
+  // tables
   ctx->hot_lds = W.hot_n > 0 && size_t(W.hot_n) * sizeof(double) <= kHotLdsMax
                      ? size_t(W.hot_n) * sizeof(double) : 0;
+  const size_t hash_lds =
+      ctx->n_hash0 > 0 ? sizeof(int) * size_t((ctx->n_hash0 + 2) & ~1) : 0;
+  const size_t tot_lds = size_t(W.tot_n) * sizeof(double);
+  ctx->ev_tot_lds = W.tot_n > 0 && tot_lds <= kTotLdsMax;
+  ctx->ev_lds = hash_lds + (ctx->ev_tot_lds ? tot_lds : 0);
+  ctx->tail_lds = ctx->hot_lds && hash_lds + tot_lds + ctx->hot_lds <= kLdsPerBlock
+                      ? hash_lds + tot_lds + ctx->hot_lds : 0;
   W.ests = DevUpload(ctx, ests);
   ctx->S = DevAlloc<double>(ctx, nb);
   ctx->S2 = DevAlloc<double>(ctx, nb);
@@ -612,21 +637,25 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
   int n_cu = 0;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) ==
           hipSuccess && n_cu > 0) {
-    const size_t lds = sizeof(int) * (ctx->n_hash0 + 1);
+    const size_t lds = ctx->ev_lds;
     int occ_a = 0, occ_c = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance, kBlock, lds) !=
-            hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide, kBlock, lds) !=
-            hipSuccess)
-      occ_a = occ_c = 2;
+    hipError_t ea, ec;
+    if (ctx->ev_tot_lds) {
+      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<true>, kBlock, lds);
+      ec = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide<true>, kBlock, lds);
+    } else {
+      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<false>, kBlock, lds);
+      ec = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide<false>, kBlock, lds);
+    }
+    if (ea != hipSuccess || ec != hipSuccess) occ_a = occ_c = 2;
     ctx->lds_grid = uint32_t(std::max(1, std::min(occ_a, occ_c))) * uint32_t(n_cu);
     // the tail kernel's lanes take the next particle when one ends: launch
     // only the blocks that are resident at once
     int occ_t = 0;
     const hipError_t et =
-        ctx->hot_lds
+        ctx->tail_lds
             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<true>, kBlock,
-                                                           ctx->hot_lds)
+                                                           ctx->tail_lds)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<false>, kBlock, 0);
     if (et != hipSuccess) occ_t = 1;
     ctx->tail_grid = uint32_t(std::max(1, occ_t)) * uint32_t(n_cu);
@@ -819,20 +848,27 @@ static void ProfCollect(mmc_ctx* ctx) {
 }
 
 static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
-  const size_t lds = ctx->W.energy_mode == 1 && ctx->n_hash0 > 0
-                         ? sizeof(int) * (ctx->n_hash0 + 1) : 0;
+  const size_t lds = ctx->ev_lds;
   const uint32_t sgrid = lds ? std::min(grid, ctx->lds_grid) : grid;
   ProfBegin(ctx, MMC_K_ADVANCE);
-  hipLaunchKernelGGL(KAdvance, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
-                     iter, qin);
+  if (ctx->ev_tot_lds)
+    hipLaunchKernelGGL(KAdvance<true>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
+                       ctx->R, iter, qin);
+  else
+    hipLaunchKernelGGL(KAdvance<false>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
+                       ctx->R, iter, qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_CROSS);
   hipLaunchKernelGGL(KCross, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R, iter,
                      qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_COLLIDE);
-  hipLaunchKernelGGL(KCollide, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W, ctx->R,
-                     iter, qin);
+  if (ctx->ev_tot_lds)
+    hipLaunchKernelGGL(KCollide<true>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
+                       ctx->R, iter, qin);
+  else
+    hipLaunchKernelGGL(KCollide<false>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
+                       ctx->R, iter, qin);
   ProfEnd(ctx);
   if (ctx->has_tsl) {
     if (ctx->opt.tsl_mode == MMC_TSL_SURROGATE && ctx->mlp.ready) {
@@ -926,9 +962,9 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     if (tail_ok && bound < tail_at && iter > 0) {
       ProfBegin(ctx, MMC_K_TAIL);
       const dim3 tgrid(std::min(Blocks(bound), ctx->tail_grid));
-      if (ctx->hot_lds)
-        hipLaunchKernelGGL(KTail<true>, tgrid, dim3(kBlock), ctx->hot_lds, ctx->stream, ctx->W,
-                           ctx->R, iter, qin);
+      if (ctx->tail_lds)
+        hipLaunchKernelGGL(KTail<true>, tgrid, dim3(kBlock), ctx->tail_lds, ctx->stream,
+                           ctx->W, ctx->R, iter, qin);
       else
         hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                            iter, qin);
@@ -1555,7 +1591,7 @@ __global__ void __launch_bounds__(256)
               double* alpha, double* beta, uint32_t* err) {
   extern __shared__ double lds_hot[];
   if (blockIdx.x * blockDim.x >= n) return;
-  const double* H = kLds ? StageHot(W, lds_hot) : W.hot;
+  const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
   const DTsl t = W.tsl[tsl];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {

@@ -108,3 +108,31 @@ def test_ce_slab_leakage_matches_oracle():
         assert np.all(S2t >= St - 1e-9)
     c = t.counters()
     assert c["histories"] == n and c["collisions"] > 10 * n
+
+
+def test_two_grid_slab_matches_oracle():
+    """Two nuclides on different energy grids, each in half of a slab: the
+    event kernels stage only the first grid's bucket hash in LDS, so the
+    second nuclide's lookups must use its own hash."""
+    pr = configs.c2_slab(n_points=60000, n_res=200)
+    other = synth.ce_nuclide(n=24000, n_res=80, seed=11, e_min=2e-11, e_max=19.0)
+    pr.nuclides.append(configs.ce_tables(other, awr=25.0))
+    s_mid = pr.add_surface("mid", abi.MMC_PLANEX, a=0.5)
+    # split the slab cell at x = 0.5: left keeps nuclide 0, right gets nuclide 1
+    m1 = pr.add_material("other", pr.materials[0]["aden"], [(1, 1.0)])
+    s0, s1 = 0, 1  # the slab's left and right planes
+    pr.cells[0] = dict(pr.cells[0], faces=[(s0, False), (s_mid, True)])
+    pr.add_cell("right", m1, [(s_mid, False), (s1, True)])
+    n = 20000
+    o = Oracle(pr)
+    o.run_fixed_source(n, threads=8)
+    t = _tracker(pr, max_in_flight=1 << 13)
+    t.run_fixed_source(n)
+    tot_o = sum(o.tallies(e)[0].sum() for e in range(2))
+    tot_t = sum(t.tallies(e)[0].sum() for e in range(2))
+    assert abs(tot_o - tot_t) <= 0.01 * n
+    for e in range(2):
+        So, _ = o.tallies(e)
+        St, _ = t.tallies(e)
+        go, gt = So[1:301].reshape(10, 30).sum(1), St[1:301].reshape(10, 30).sum(1)
+        assert np.all(np.abs(go - gt) <= 4 * np.sqrt(go + gt + 1) + 2)
