@@ -7,12 +7,11 @@
 //   inverse-CDF arithmetic as SampleBeta/SampleAlpha, with linear
 //   interpolation between grid energies / betas instead of a random pick.
 // - Training: FP32 tiled GEMMs on the vector ALUs, SiLU, MSE loss, Adam.
-// - Inference: one fused kernel per batch.  Layer 0 (4 -> 256) and layer 3
-//   (256 -> 2) run on the vector ALUs; the two 256 x 256 layers run on
-//   v_mfma_f32_32x32x16_bf16 with the weights staged in LDS.  Each layer is
-//   computed transposed (Z^T = W H^T) so the accumulator layout of one layer
-//   is the B-operand layout of the next once the next layer's input
-//   features are permuted; activations never leave registers.
+// - Inference: one fused persistent kernel (KMlpF).  Layer 0 (4 -> 256) and
+//   layer 3 (256 -> 2) run on the vector ALUs; the two 256 x 256 layers run
+//   on v_mfma_f32_32x32x16_bf16 with each wave's share of the weights held
+//   in registers for the whole launch and the activations passed between
+//   layers through LDS.  Each layer is computed transposed (Z^T = W H^T).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +37,6 @@ namespace mmc {
 
 constexpr int kHid = 256;           // hidden width of the fused kernel
 constexpr int kFragElems = 8 * 16 * 64 * 8;  // bf16 per 256x256 layer (65536)
-constexpr int kStage = 8;  // weight staging depth (16-byte loads per thread)
 
 // The fused kernel's small parameters, FP32, one block staged in LDS:
 //   [kW0, +1024)  layer-0 weights as float4 [h 2][kb 16][q 8]
@@ -77,59 +75,21 @@ __device__ __forceinline__ float SiluFast(float z) {
 
 void MlpFree(MlpState& m) {
   void* ps[] = {m.params, m.wbf16, m.adam_m, m.adam_v, m.grads, m.act, m.loss,
-                m.xin, m.yout, m.fb, m.packed, m.h1};
+                m.xin, m.yout, m.fb, m.packed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   m = MlpState{};
 }
 
 // ---------------------------------------------------------------------------
-// BF16 MFMA inference in two persistent kernels, one per 256 x 256 layer, so
-// that a block stages its layer's weights in LDS once per launch:
-//   KMlpA: inputs -> layer 0 (VALU) -> layer 1 (MFMA) -> H1, the layer-1
-//          activations in BF16, stored in layer 2's B-fragment order
-//          ([32-row tile][kb 16][lane 64] x 8 bf16)
-//   KMlpB: H1 -> layer 2 (MFMA) -> layer 3 (VALU) -> outputs
-// Each wave owns whole 32-row tiles and runs without block barriers after
-// the staging; a block holds kMlpWaves waves (two per SIMD, so that one
-// wave's matrix instructions overlap another's vector work).  In the tracker
-// KMlpA gathers (ln E, T, u1, u2) of the thermal-scattering queue and KMlpB
-// applies the scatter (TrackIO below).  The rare rows the network cannot
-// serve wait in a fallback queue that the POD sampler empties once per batch
-// of iterations, so that its latency is not paid every iteration.
+// BF16 MFMA inference (KMlpF below): a block holds kMlpWaves waves, two per
+// SIMD, so that one wave's matrix instructions overlap another's vector work.
+// In the tracker the kernel gathers (ln E, T, u1, u2) of the thermal-
+// scattering queue itself and applies the scatter.  The rare rows the
+// network cannot serve wait in a fallback queue that the POD sampler empties
+// once per batch of iterations, so that its latency is not paid every
+// iteration.
 constexpr int kMlpWaves = 8;
-constexpr uint32_t kTileBf16x8 = 16 * 64;  // bf16x8 per 32-row tile of H1
-
-// Stages one layer's A fragments and the small parameters (block-wide).
-__device__ __forceinline__ void StageLayer(bf16x8* lds, float* sp, const bf16x8* src,
-                                           const float* small) {
-  for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
-    reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(small)[i];
-  for (int i0 = threadIdx.x; i0 < kFragElems / 8; i0 += kStage * blockDim.x) {
-    bf16x8 v[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; u++) v[u] = src[i0 + u * blockDim.x];
-#pragma unroll
-    for (int u = 0; u < kStage; u++) lds[i0 + u * blockDim.x] = v[u];
-  }
-  __syncthreads();
-}
-// acc = W_layer (from LDS) x B over the 256 input features.
-__device__ __forceinline__ void MfmaLayer(const bf16x8* lds, int lane, const bf16x8 B[16],
-                                          f32x16 acc[8]) {
-#pragma unroll
-  for (int t = 0; t < 8; t++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
-#pragma unroll
-  for (int kb = 0; kb < 16; kb++) {
-#pragma unroll
-    for (int t = 0; t < 8; t++) {
-      const bf16x8 a = lds[(t * 16 + kb) * 64 + lane];
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kb], acc[t], 0, 0, 0);
-    }
-  }
-}
 
 // Returns false when the network's outgoing energy leaves the trained box
 // (E' = E + beta kT below e_min, which includes E' <= 0, or not a number):
@@ -167,125 +127,190 @@ __device__ __forceinline__ bool InBoxT(const TrackIO& io, double E, double T) {
   return E >= io.e_min && E <= io.e_max && T >= io.t_lo && T <= io.t_hi;
 }
 
-template <bool kTrack>
-__global__ void __launch_bounds__(64 * kMlpWaves)
-    KMlpA(MlpDev m, const float* __restrict__ x, const uint32_t* n_ptr, uint32_t n_static,
-          bf16x8* __restrict__ h1, TrackIO io) {
-  extern __shared__ bf16x8 lds[];  // kFragElems bf16 = 128 KiB, then kSmall floats
-  float* sp = reinterpret_cast<float*>(lds + kFragElems / 8);
-  const uint32_t n = n_ptr ? *n_ptr : n_static;
-  const uint32_t n_tiles = (n + 31) / 32;
-  if (blockIdx.x * kMlpWaves >= n_tiles) return;  // no rows: stage nothing
-  StageLayer(lds, sp, m.frag1, m.small);
-  const int wave = threadIdx.x / 64;
-  const int lane = threadIdx.x % 64;
-  const int h = lane / 32;
-  const int j = lane % 32;
-  const float4* w0 = reinterpret_cast<const float4*>(sp + kW0) + h * 128;
-  const float* b0 = sp + kB0 + h * 128;
-  const float* bias = sp + kB1 + h * 128;
-  for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < n_tiles;
-       tile += gridDim.x * kMlpWaves) {
-    // compiler-only memory barrier: the weights in LDS are re-read every
-    // tile instead of being hoisted into (and spilled from) registers
-    __asm__ volatile("" ::: "memory");
-    const uint32_t row = tile * 32 + j;
-    float xr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (row < n) {
-      if (kTrack) {
-        // KTslGather of the event loop: (ln E, T, u1, u2) of queued particle p
-        const uint32_t p = io.R.QT[row];
-        const double E = io.R.B.E[p];
-        const double T = io.R.B.tslT[p];
-        uint32_t g = io.R.B.rng[p];
-        const double u1 = Canonical(g), u2 = Canonical(g);
-        if (h == 0) io.R.B.rng[p] = g;
-        xr[0] = float(log(E));
-        xr[1] = float(T);
-        xr[2] = float(u1);
-        xr[3] = float(u2);
-      } else {
-        const float4 v = reinterpret_cast<const float4*>(x)[row];
-        xr[0] = v.x;
-        xr[1] = v.y;
-        xr[2] = v.z;
-        xr[3] = v.w;
-      }
+// ---------------------------------------------------------------------------
+// Fused inference: both 256 x 256 layers in one persistent kernel, with the
+// weights held in registers instead of LDS.  Wave w of a block owns output
+// features 32w .. 32w + 31 of layers 1 and 2 and keeps its 16 A fragments of
+// each (128 registers) for the whole launch.  The activations of a block's
+// kFRows rows pass between layers through LDS, stored in the K-permuted
+// order of the B fragments (position 16 kb + 8 h + q holds input feature
+// FeatB(kb, h, q)), so that a B fragment is one 16-byte LDS read and an
+// accumulator quad one 8-byte LDS write.  One iteration of a block:
+//   gather (waves 0 .. kFRows/64 - 1)          -> X      | barrier
+//   layer 0 on the vector ALUs, every wave      -> H0     | barrier
+//   layer 1: W1 slice x H0, bias, SiLU          -> H1     | barrier
+//   layer 2: W2 slice x H1, bias, SiLU, then the wave's share of
+//   layer 3 (its 32 features)                   -> parts  | barrier
+//   finish (waves 0 .. kFRows/64 - 1): the sum of the 8 shares, then the
+//   output rows or, in the tracker, the scatter.
+// The layer-1 activations never go to memory, and a block reads each weight
+// once per launch.
+constexpr int kFRows = 64;         // rows per block iteration
+constexpr int kFCb = kFRows / 32;  // 32-row column blocks per wave
+constexpr int kHStride = 264;      // bf16 per LDS activation row (256 + 8 padding)
+static_assert(kMlpWaves * 32 == kHid, "one wave per 32 output features");
+static_assert(kFRows % 64 == 0 && (64 * kMlpWaves) % kFRows == 0, "row tiling");
+// LDS: small parameters | X [kFRows] float4 | parts [waves][kFRows] float2 |
+// pidx [kFRows] | H0, H1 [kFRows][kHStride] bf16
+constexpr size_t kFusedLds = sizeof(float) * (kSmall + 4 * kFRows + 2 * kMlpWaves * kFRows) +
+                             sizeof(uint32_t) * kFRows +
+                             sizeof(__bf16) * 2 * kFRows * kHStride;
+static_assert(kFusedLds <= 160 * 1024, "LDS of one CU");
+
+// LDS position (B-fragment order) of the feature that accumulator register r
+// of output tile t holds in lane half h, FeatC(t, r, h) = FeatB(2t + j/2, h,
+// 4 (j % 2) + r % 4) with j = r / 4.
+__host__ __device__ __forceinline__ int PosC(int t, int r, int h) {
+  const int j = r / 4;
+  return 16 * (2 * t + j / 2) + 8 * h + 4 * (j % 2) + r % 4;
+}
+// acc[c] = A (this wave's 32 output features) x H rows 32c .. 32c + 31.
+__device__ __forceinline__ void MfmaRows(const __bf16* H, const bf16x8 A[16], int h, int j,
+                                         f32x16 acc[kFCb]) {
+#pragma unroll
+  for (int c = 0; c < kFCb; c++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[c][r] = 0.0f;
+#pragma unroll
+  for (int kb = 0; kb < 16; kb++) {
+#pragma unroll
+    for (int c = 0; c < kFCb; c++) {
+      const bf16x8 b =
+          *reinterpret_cast<const bf16x8*>(H + (32 * c + j) * kHStride + 16 * kb + 8 * h);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[kb], b, acc[c], 0, 0, 0);
     }
-    float xn[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) xn[k] = (xr[k] - m.in_shift[k]) * m.in_scale[k];
-    // layer 0 on the vector ALUs, straight into B-fragment order
-    bf16x8 B[16];
-#pragma unroll
-    for (int kb = 0; kb < 16; kb++) {
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const float4 w = w0[kb * 8 + q];
-        const float z = b0[kb * 8 + q] + w.x * xn[0] + w.y * xn[1] + w.z * xn[2] + w.w * xn[3];
-        B[kb][q] = (__bf16)SiluFast(z);
-      }
-    }
-    f32x16 acc[8];
-    MfmaLayer(lds, lane, B, acc);
-    // bias + SiLU, rounded to BF16 in layer 2's B-fragment order
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-#pragma unroll
-      for (int r = 0; r < 16; r++)
-        B[2 * t + r / 8][r % 8] = (__bf16)SiluFast(acc[t][r] + bias[t * 16 + r]);
-    bf16x8* out = h1 + size_t(tile) * kTileBf16x8 + lane;
-#pragma unroll
-    for (int kb = 0; kb < 16; kb++) out[kb * 64] = B[kb];
   }
 }
 
 template <bool kTrack>
 __global__ void __launch_bounds__(64 * kMlpWaves)
-    KMlpB(MlpDev m, float* __restrict__ y, const uint32_t* n_ptr, uint32_t n_static,
-          const bf16x8* __restrict__ h1, TrackIO io) {
-  extern __shared__ bf16x8 lds[];
-  float* sp = reinterpret_cast<float*>(lds + kFragElems / 8);
+    KMlpF(MlpDev m, const float* __restrict__ x, float* __restrict__ y, const uint32_t* n_ptr,
+          uint32_t n_static, TrackIO io) {
+  extern __shared__ float4 lds_f4[];
+  float* sp = reinterpret_cast<float*>(lds_f4);
+  float4* X = reinterpret_cast<float4*>(sp + kSmall);
+  float2* parts = reinterpret_cast<float2*>(X + kFRows);
+  uint32_t* pidx = reinterpret_cast<uint32_t*>(parts + kMlpWaves * kFRows);
+  __bf16* H0 = reinterpret_cast<__bf16*>(pidx + kFRows);
+  __bf16* H1 = H0 + kFRows * kHStride;
   const uint32_t n = n_ptr ? *n_ptr : n_static;
-  const uint32_t n_tiles = (n + 31) / 32;
-  if (blockIdx.x * kMlpWaves >= n_tiles) return;
-  StageLayer(lds, sp, m.frag2, m.small);
+  const uint32_t n_it = (n + kFRows - 1) / kFRows;
+  if (blockIdx.x >= n_it) return;  // no rows: read nothing
   const int wave = threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   const int h = lane / 32;
   const int j = lane % 32;
-  const float* bias = sp + kB2 + h * 128;
-  const float2* w3 = reinterpret_cast<const float2*>(sp + kW3) + h * 128;
-  for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < n_tiles;
-       tile += gridDim.x * kMlpWaves) {
-    // compiler-only memory barrier: the weights in LDS are re-read every
-    // tile instead of being hoisted into (and spilled from) registers
-    __asm__ volatile("" ::: "memory");
-    const uint32_t row = tile * 32 + j;
-    const bool valid = row < n;
-    bf16x8 B[16];
-    const bf16x8* in = h1 + size_t(tile) * kTileBf16x8 + lane;
+  for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
+    reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(m.small)[i];
+  bf16x8 A1[16], A2[16];
 #pragma unroll
-    for (int kb = 0; kb < 16; kb++) B[kb] = in[kb * 64];
-    f32x16 acc[8];
-    MfmaLayer(lds, lane, B, acc);
-    // bias + SiLU, then layer 3 (256 -> 2) in FP32, lane halves combined
-    float o0 = 0.0f, o1 = 0.0f;
+  for (int kb = 0; kb < 16; kb++) {
+    A1[kb] = m.frag1[(wave * 16 + kb) * 64 + lane];
+    A2[kb] = m.frag2[(wave * 16 + kb) * 64 + lane];
+  }
+  const float* bias1 = sp + kB1 + h * 128 + wave * 16;
+  const float* bias2 = sp + kB2 + h * 128 + wave * 16;
+  const float2* w3 = reinterpret_cast<const float2*>(sp + kW3) + h * 128 + wave * 16;
+  const float4* w0 = reinterpret_cast<const float4*>(sp + kW0);
+  const float* b0 = sp + kB0;
+  const bool io_wave = wave < kFRows / 64;
+  for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
+    // gather: (ln E, T, u1, u2) of the queued particle, or the input row
+    if (io_wave) {
+      const int lr = wave * 64 + lane;
+      const uint32_t row = it * kFRows + lr;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      uint32_t p = 0;
+      if (row < n) {
+        if (kTrack) {
+          // KTslGather of the event loop
+          p = io.R.QT[row];
+          const double E = io.R.B.E[p];
+          const double T = io.R.B.tslT[p];
+          uint32_t g = io.R.B.rng[p];
+          const double u1 = Canonical(g), u2 = Canonical(g);
+          io.R.B.rng[p] = g;
+          v = make_float4(float(log(E)), float(T), float(u1), float(u2));
+        } else {
+          v = reinterpret_cast<const float4*>(x)[row];
+        }
+      }
+      X[lr] = make_float4((v.x - m.in_shift[0]) * m.in_scale[0],
+                          (v.y - m.in_shift[1]) * m.in_scale[1],
+                          (v.z - m.in_shift[2]) * m.in_scale[2],
+                          (v.w - m.in_shift[3]) * m.in_scale[3]);
+      if (kTrack) pidx[lr] = p;
+    }
+    __syncthreads();
+    // layer 0 (4 -> 256): thread = (row, 64 consecutive positions)
+    {
+      const int lr = threadIdx.x % kFRows;
+      const int c0 = threadIdx.x / kFRows;
+      const float4 xn = X[lr];
 #pragma unroll
-    for (int t = 0; t < 8; t++)
+      for (int g = 0; g < (256 * kFRows) / (64 * kMlpWaves) / 8; g++) {
+        const int P0 = (256 * kFRows / (64 * kMlpWaves)) * c0 + 8 * g;
+        const int kb = P0 / 16, hh = (P0 % 16) / 8;
+        const int i0 = (hh * 16 + kb) * 8;  // packed index of FeatB(kb, hh, 0..7)
+        bf16x8 v;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const float4 w = w0[i0 + q];
+          const float z = b0[i0 + q] + w.x * xn.x + w.y * xn.y + w.z * xn.z + w.w * xn.w;
+          v[q] = (__bf16)SiluFast(z);
+        }
+        *reinterpret_cast<bf16x8*>(H0 + lr * kHStride + P0) = v;
+      }
+    }
+    __syncthreads();
+    f32x16 acc[kFCb];
+    // layer 1 -> H1
+    MfmaRows(H0, A1, h, j, acc);
+#pragma unroll
+    for (int c = 0; c < kFCb; c++)
+#pragma unroll
+      for (int jq = 0; jq < 4; jq++) {
+        typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+        bf16x4 v;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          v[e] = (__bf16)SiluFast(acc[c][4 * jq + e] + bias1[4 * jq + e]);
+        *reinterpret_cast<bf16x4*>(H1 + (32 * c + j) * kHStride + PosC(wave, 4 * jq, h)) = v;
+      }
+    __syncthreads();
+    // layer 2, then this wave's share of layer 3 (256 -> 2) in FP32
+    MfmaRows(H1, A2, h, j, acc);
+#pragma unroll
+    for (int c = 0; c < kFCb; c++) {
+      float o0 = 0.0f, o1 = 0.0f;
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const float a = SiluFast(acc[t][r] + bias[t * 16 + r]);
-        const float2 w = w3[t * 16 + r];
+        const float a = SiluFast(acc[c][r] + bias2[r]);
+        const float2 w = w3[r];
         o0 += w.x * a;
         o1 += w.y * a;
       }
-    o0 += __shfl_xor(o0, 32);
-    o1 += __shfl_xor(o1, 32);
+      o0 += __shfl_xor(o0, 32);
+      o1 += __shfl_xor(o1, 32);
+      if (h == 0) parts[wave * kFRows + 32 * c + j] = make_float2(o0, o1);
+    }
+    __syncthreads();
+    if (!io_wave) continue;
+    // finish
+    const int lr = wave * 64 + lane;
+    const uint32_t row = it * kFRows + lr;
+    const bool valid = row < n;
+    float o0 = 0.0f, o1 = 0.0f;
+#pragma unroll
+    for (int w = 0; w < kMlpWaves; w++) {
+      const float2 s = parts[w * kFRows + lr];
+      o0 += s.x;
+      o1 += s.y;
+    }
     const float y1 = (o0 + m.b3[0]) * m.out_scale[0] + m.out_shift[0];
     const float y2 = (o1 + m.b3[1]) * m.out_scale[1] + m.out_shift[1];
     if (!kTrack) {
-      if (valid && h == 0) {
+      if (valid) {
         y[size_t(row) * 2 + 0] = y1;
         y[size_t(row) * 2 + 1] = y2;
       }
@@ -296,9 +321,8 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     // parked in the fallback queue for the POD sampler (MlpFlushFallback)
     const RunState& R = io.R;
     bool alive = false, fall = false;
-    uint32_t p = 0;
-    if (valid && h == 0) {
-      p = R.QT[row];
+    const uint32_t p = pidx[lr];
+    if (valid) {
       double E = R.B.E[p];
       const double T = R.B.tslT[p];
       const DTsl t = io.W.tsl[R.B.tslI[p]];
@@ -325,7 +349,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     if (alive) R.Q[io.qin ^ 1][iq] = p;
     const uint32_t ifb = WavePush(R.misc + kMiscFallback, fall, tot);
     if (fall) io.fb[ifb] = p;
-    WaveCount(R.ev + kEvTsl, valid && h == 0);
+    WaveCount(R.ev + kEvTsl, valid);
     WaveCount(R.ev + kEvFallback, fall);
   }
 }
@@ -626,54 +650,30 @@ static MlpDev DevView(const MlpState& m) {
   return d;
 }
 
-static constexpr size_t kMlpLds = kFragElems * sizeof(__bf16) + kSmall * sizeof(float);
-static uint32_t MlpGrid(uint64_t rows) {
-  const uint64_t blocks = ((rows + 31) / 32 + kMlpWaves - 1) / kMlpWaves;
-  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(blocks, 256)));  // <= 1 per CU
-}
-// Batch inference of n rows (h1 holds at least n rows).
-static void LaunchInfer(const MlpState& m, const float* x, float* y, uint32_t n, bf16x8* h1,
-                        hipStream_t s) {
+// Batch inference of n rows.
+static void LaunchInfer(const MlpState& m, const float* x, float* y, uint32_t n, hipStream_t s) {
   const TrackIO none{};
-  hipLaunchKernelGGL(KMlpA<false>, dim3(MlpGrid(n)), dim3(64 * kMlpWaves), kMlpLds, s,
-                     DevView(m), x, nullptr, n, h1, none);
-  hipLaunchKernelGGL(KMlpB<false>, dim3(MlpGrid(n)), dim3(64 * kMlpWaves), kMlpLds, s,
-                     DevView(m), y, nullptr, n, h1, none);
-}
-// H1 rows available for batch inference (grown on demand, at most 2^20 rows
-// per chunk).
-static int EnsureH1(mmc_ctx* ctx, MlpState& m, uint64_t rows) {
-  rows = std::min<uint64_t>(rows, 1u << 20);
-  if (m.h1_rows >= rows) return MMC_OK;
-  if (m.h1) (void)hipFree(m.h1);
-  m.h1 = nullptr;
-  m.h1_rows = 0;
-  const uint64_t tiles = (rows + 31) / 32;
-  if (hipMalloc(&m.h1, tiles * kTileBf16x8 * sizeof(bf16x8)) != hipSuccess)
-    return MmcFail(ctx, MMC_ERR_HIP, "surrogate activation buffer allocation failed");
-  m.h1_rows = uint32_t(tiles * 32);
-  return MMC_OK;
+  const uint32_t grid = uint32_t(std::min<uint64_t>((uint64_t(n) + kFRows - 1) / kFRows, 256));
+  hipLaunchKernelGGL(KMlpF<false>, dim3(std::max(grid, 1u)), dim3(64 * kMlpWaves), kFusedLds,
+                     s, DevView(m), x, y, nullptr, n, none);
 }
 static int InferChunked(mmc_ctx* ctx, MlpState& m, const float* x, float* y, uint64_t n) {
-  const int rc = EnsureH1(ctx, m, n);
-  if (rc) return rc;
-  for (uint64_t c0 = 0; c0 < n; c0 += m.h1_rows) {
-    const uint32_t c = uint32_t(std::min<uint64_t>(n - c0, m.h1_rows));
-    LaunchInfer(m, x + c0 * 4, y + c0 * 2, c, static_cast<bf16x8*>(m.h1), MmcStream(ctx));
+  constexpr uint64_t kChunk = 1ull << 30;  // rows per launch (32-bit row indices)
+  for (uint64_t c0 = 0; c0 < n; c0 += kChunk) {
+    const uint32_t c = uint32_t(std::min<uint64_t>(n - c0, kChunk));
+    LaunchInfer(m, x + c0 * 4, y + c0 * 2, c, MmcStream(ctx));
   }
   return MMC_OK;
 }
 
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
                uint32_t grid, hipStream_t s, void (*mark)(void*, int), void* mark_arg) {
-  if (!m.ready || !m.h1 || !m.fb) return 1;
+  if (!m.ready || !m.fb) return 1;
   TrackIO io{W, R, iter, qin, m.fb, m.e_min, m.e_max, m.t_lo, m.t_hi};
   // both kernels read the queue length themselves: one block per CU
   mark(mark_arg, MMC_K_MLP);
-  hipLaunchKernelGGL(KMlpA<true>, dim3(256), dim3(64 * kMlpWaves), kMlpLds, s, DevView(m),
-                     nullptr, Ring(R, iter, kT), 0u, static_cast<bf16x8*>(m.h1), io);
-  hipLaunchKernelGGL(KMlpB<true>, dim3(256), dim3(64 * kMlpWaves), kMlpLds, s, DevView(m),
-                     nullptr, Ring(R, iter, kT), 0u, static_cast<const bf16x8*>(m.h1), io);
+  hipLaunchKernelGGL(KMlpF<true>, dim3(256), dim3(64 * kMlpWaves), kFusedLds, s, DevView(m),
+                     nullptr, nullptr, Ring(R, iter, kT), 0u, io);
   mark(mark_arg, -1);
   return 0;
 }
@@ -1163,14 +1163,9 @@ extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const
 int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap) {
   MlpState& m = MmcMlp(ctx);
   if (m.stage_cap >= cap) return MMC_OK;
-  if (m.h1) (void)hipFree(m.h1);
   if (m.fb) (void)hipFree(m.fb);
-  m.h1 = nullptr;
   m.fb = nullptr;
-  m.h1_rows = 0;
-  const uint64_t tiles = (uint64_t(cap) + 31) / 32;
-  if (hipMalloc(&m.h1, tiles * kTileBf16x8 * sizeof(bf16x8)) != hipSuccess ||
-      hipMalloc(&m.fb, sizeof(uint32_t) * size_t(cap)) != hipSuccess)
+  if (hipMalloc(&m.fb, sizeof(uint32_t) * size_t(cap)) != hipSuccess)
     return MmcFail(ctx, MMC_ERR_HIP, "surrogate staging allocation failed");
   m.stage_cap = cap;
   return MMC_OK;

@@ -31,8 +31,6 @@ struct MlpState {
   float* yout = nullptr;          // unused
   uint32_t* fb = nullptr;         // parked fallback particles [cap]
   uint32_t stage_cap = 0;         // rows of the tracker buffers
-  void* h1 = nullptr;             // layer-1 activations (BF16 B fragments)
-  uint32_t h1_rows = 0;
 };
 
 void MlpFree(MlpState& m);
@@ -41,9 +39,9 @@ struct mmc_ctx;
 // Staging buffers of the surrogate step for `cap` particle slots.
 int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap);
 namespace mmc {
-// One thermal-scattering step of the tracker with the surrogate: two network
-// kernels; the first gathers (log E, T, u1, u2) of the TSL queue, the second
-// applies the scatter (the POD sampler serves the rows the network cannot).
+// One thermal-scattering step of the tracker with the surrogate: one network
+// kernel that gathers (log E, T, u1, u2) of the TSL queue and applies the
+// scatter (the POD sampler serves the rows the network cannot).
 // `mark(k)` is called before each launch with the profiling kind of the
 // next kernel (MMC_K_TSL or MMC_K_MLP) and with -1 after the last one.
 int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
