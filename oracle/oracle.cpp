@@ -42,6 +42,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -1760,6 +1761,55 @@ int orc_tsl_sample(orc_world* ow, int tsl, uint64_t n, const double* E,
   for (int k = 0; k < threads; k++) pool.emplace_back(worker, k);
   for (auto& th : pool) th.join();
   return ErrToStatus(ow, err.load());
+}
+
+// ---- output text (the reference's .out file) --------------------------------
+static uint64_t CopyOut(const std::string& t, char* buf, uint64_t cap) {
+  if (buf && cap > 0) {
+    const uint64_t k = std::min<uint64_t>(t.size(), cap - 1);
+    std::memcpy(buf, t.data(), k);
+    buf[k] = 0;
+  }
+  return t.size();
+}
+uint64_t orc_format_scientific(double v, char* buf, uint64_t cap) {
+  std::ostringstream os;
+  os << std::scientific << v;
+  return CopyOut(os.str(), buf, cap);
+}
+// Scorable::GetScoreAsString (Scorable.cpp:51-70)
+uint64_t orc_format_scores(const double* S, const double* S2, uint64_t n,
+                           double total_weight, char* buf, uint64_t cap) {
+  std::ostringstream os;
+  os << "mean\n----\n" << std::scientific;
+  for (uint64_t i = 0; i < n; i++) os << S[i] / total_weight << ", ";
+  os << "\n\n" << "std dev\n-------\n";
+  for (uint64_t i = 0; i < n; i++)
+    os << std::sqrt(S2[i] - S[i] * S[i] / total_weight) / total_weight << ", ";
+  os << "\n\n";
+  return CopyOut(os.str(), buf, cap);
+}
+// NoBins / LinspaceBins / LogspaceBins / BoundaryBins::to_string
+// (Bins.cpp:51, :84-90, :125-131, :164-170); n is the user's bin count
+// (the reference adds the two overflow bins).
+uint64_t orc_format_bins(int kind, int n, double lo, double hi, double base,
+                         const double* bounds, char* buf, uint64_t cap) {
+  std::ostringstream os;
+  if (kind == MMC_BINS_NONE) {
+    os << "none";
+  } else if (kind == MMC_BINS_BOUNDARIES) {
+    for (int i = 0; i < n; i++) os << std::scientific << bounds[i] << ", ";
+  } else {
+    const size_t n_bins = size_t(n) + 2;
+    const double width = (hi - lo) / double(n_bins - 2);
+    for (size_t i = 0; i < n_bins - 1; i++) {
+      if (kind == MMC_BINS_LINSPACE)
+        os << std::scientific << lo + double(i) * width << ", ";
+      else
+        os << std::scientific << std::pow(base, lo + double(i) * width) << ", ";
+    }
+  }
+  return CopyOut(os.str(), buf, cap);
 }
 
 }  // extern "C"

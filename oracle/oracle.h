@@ -61,6 +61,20 @@ int orc_tsl_total(orc_world* w, int tsl, double E, double T, double* out);
 int orc_set_entropy_mesh(orc_world* w, const int32_t n[3], const double lo[3],
                          const double hi[3]);
 
+/* Output text, written with the C++ streams the reference writes with
+ * (std::scientific, default precision).  Each returns the length of the
+ * text (without the terminating zero) and writes at most cap bytes
+ * including it. */
+/* One number as `os << std::scientific << v` prints it. */
+uint64_t orc_format_scientific(double v, char* buf, uint64_t cap);
+/* Scorable::GetScoreAsString (Scorable.cpp:51-70). */
+uint64_t orc_format_scores(const double* S, const double* S2, uint64_t n,
+                           double total_weight, char* buf, uint64_t cap);
+/* Bins::to_string of one bins spec (Bins.cpp:51, :84-90, :125-131,
+ * :164-170); kind as mmc_bins.kind, bounds for MMC_BINS_BOUNDARIES. */
+uint64_t orc_format_bins(int kind, int n, double lo, double hi, double base,
+                         const double* bounds, char* buf, uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

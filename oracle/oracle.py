@@ -62,8 +62,50 @@ def lib():
         L.orc_rng_exponentials.argtypes = [C.c_uint64, C.c_double, C.c_uint64,
                                            abi.c_double_p]
         L.orc_rng_directions.argtypes = [C.c_uint64, C.c_uint64, abi.c_double_p]
+        L.orc_format_scientific.argtypes = [C.c_double, C.c_char_p, C.c_uint64]
+        L.orc_format_scientific.restype = C.c_uint64
+        L.orc_format_scores.argtypes = [abi.c_double_p, abi.c_double_p, C.c_uint64,
+                                        C.c_double, C.c_char_p, C.c_uint64]
+        L.orc_format_scores.restype = C.c_uint64
+        L.orc_format_bins.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      abi.c_double_p, C.c_char_p, C.c_uint64]
+        L.orc_format_bins.restype = C.c_uint64
         _lib = L
     return _lib
+
+
+def _text(call):
+    n = call(None, 0)
+    buf = C.create_string_buffer(int(n) + 1)
+    call(buf, n + 1)
+    return buf.value.decode()
+
+
+def format_scientific(v):
+    """`os << std::scientific << v` with libstdc++."""
+    return _text(lambda b, c: lib().orc_format_scientific(float(v), b, c))
+
+
+def format_scores(S, S2, total_weight):
+    """Scorable::GetScoreAsString with libstdc++."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    S2 = np.ascontiguousarray(S2, dtype=np.float64)
+    return _text(lambda b, c: lib().orc_format_scores(abi.dptr(S), abi.dptr(S2), len(S),
+                                                      float(total_weight), b, c))
+
+
+def format_bins(spec):
+    """Bins::to_string of a bins spec (dict as in Problem.estimators, or None)."""
+    if not spec:
+        return _text(lambda b, c: lib().orc_format_bins(abi.MMC_BINS_NONE, 0, 0.0, 0.0, 0.0,
+                                                        None, b, c))
+    if spec["kind"] == abi.MMC_BINS_BOUNDARIES:
+        bd = np.ascontiguousarray(spec["bounds"], dtype=np.float64)
+        return _text(lambda b, c: lib().orc_format_bins(spec["kind"], len(bd), 0.0, 0.0, 0.0,
+                                                        abi.dptr(bd), b, c))
+    return _text(lambda b, c: lib().orc_format_bins(
+        spec["kind"], int(spec["bins"]), float(spec["min"]), float(spec["max"]),
+        float(spec.get("base", 10.0)), None, b, c))
 
 
 def rng_draws(seed, n):

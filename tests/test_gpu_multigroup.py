@@ -183,3 +183,33 @@ def test_keigenvalue_slab_matches_oracle():
     St, _ = t.tallies(0)
     assert St.sum() > 0
     assert abs(St.sum() - So.sum()) / So.sum() < 0.05
+
+
+def test_out_file_of_the_reference_input(golden, tmp_path):
+    """The reference's point-source leakage input through the GPU tracker,
+    written as the reference's .out file: the same text as the oracle's run
+    apart from the last digits of the numbers, and means equal to S / N."""
+    require_gpu()
+    from minimc_amd import output
+    from minimc_amd.tracker import Tracker
+    pr = load_xml(os.path.join(golden, "point_source_leakage.xml"))
+    o = Oracle(pr)
+    o.run_fixed_source(pr.histories, threads=8)
+    t = Tracker(pr, max_in_flight=1 << 12)
+    t.run_fixed_source(pr.histories)
+    n_est = len(pr.estimators)
+    to = output.out_text(pr, [o.tallies(e) for e in range(n_est)], pr.histories)
+    tg = output.out_text(pr, [t.tallies(e) for e in range(n_est)], pr.histories)
+    lo, lg = to.split("\n"), tg.split("\n")
+    assert len(lo) == len(lg)
+    for a, b in zip(lo, lg):
+        if "e+" in a or "e-" in a:
+            va = [float(x) for x in a.split(",") if x.strip()]
+            vb = [float(x) for x in b.split(",") if x.strip()]
+            assert len(va) == len(vb)
+            assert np.allclose(va, vb, rtol=0.05, atol=3.0 / pr.histories)
+        else:
+            assert a == b
+    path = tmp_path / "point_source_leakage.out"
+    output.write_out(str(path), pr, [t.tallies(e) for e in range(n_est)], pr.histories)
+    assert path.read_text() == tg
