@@ -182,6 +182,21 @@ __device__ __forceinline__ void MfmaRows(const __bf16* H, const bf16x8 A[16], in
   }
 }
 
+// (ln E, T, u1, u2) of queued particle p, with u1 and u2 drawn from its
+// stream (the gather of the event loop); advances the stored RNG state.
+__device__ __forceinline__ float4 TslInputs(const RunState& R, uint32_t p) {
+  const double E = R.B.E[p];
+  const double T = R.B.tslT[p];
+  uint32_t g = R.B.rng[p];
+  const double u1 = Canonical(g), u2 = Canonical(g);
+  R.B.rng[p] = g;
+  return make_float4(float(log(E)), float(T), float(u1), float(u2));
+}
+__device__ __forceinline__ float4 Normalize(const MlpDev& m, float4 v) {
+  return make_float4((v.x - m.in_shift[0]) * m.in_scale[0], (v.y - m.in_shift[1]) * m.in_scale[1],
+                     (v.z - m.in_shift[2]) * m.in_scale[2], (v.w - m.in_shift[3]) * m.in_scale[3]);
+}
+
 template <bool kTrack>
 __global__ void __launch_bounds__(64 * kMlpWaves)
     KMlpF(MlpDev m, const float* __restrict__ x, float* __restrict__ y, const uint32_t* n_ptr,
@@ -214,32 +229,48 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
   const float4* w0 = reinterpret_cast<const float4*>(sp + kW0);
   const float* b0 = sp + kB0;
   const bool io_wave = wave < kFRows / 64;
-  for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
-    // gather: (ln E, T, u1, u2) of the queued particle, or the input row
-    if (io_wave) {
-      const int lr = wave * 64 + lane;
-      const uint32_t row = it * kFRows + lr;
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      uint32_t p = 0;
-      if (row < n) {
-        if (kTrack) {
-          // KTslGather of the event loop
-          p = io.R.QT[row];
-          const double E = io.R.B.E[p];
-          const double T = io.R.B.tslT[p];
-          uint32_t g = io.R.B.rng[p];
-          const double u1 = Canonical(g), u2 = Canonical(g);
-          io.R.B.rng[p] = g;
-          v = make_float4(float(log(E)), float(T), float(u1), float(u2));
-        } else {
-          v = reinterpret_cast<const float4*>(x)[row];
-        }
+  const int lr_io = wave * 64 + lane;  // the row an io wave's lane gathers and finishes
+  const RunState& R = io.R;
+  // The first iteration's inputs.  Later iterations' inputs are loaded
+  // while the previous iteration computes (software pipelining): the queue
+  // entry before layer 0, the particle state before layer 1, and X and pidx
+  // are written after the finish.
+  if (io_wave) {
+    const uint32_t row = blockIdx.x * kFRows + lr_io;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t p = 0;
+    if (row < n) {
+      if (kTrack) {
+        p = R.QT[row];
+        v = TslInputs(R, p);
+      } else {
+        v = reinterpret_cast<const float4*>(x)[row];
       }
-      X[lr] = make_float4((v.x - m.in_shift[0]) * m.in_scale[0],
-                          (v.y - m.in_shift[1]) * m.in_scale[1],
-                          (v.z - m.in_shift[2]) * m.in_scale[2],
-                          (v.w - m.in_shift[3]) * m.in_scale[3]);
-      if (kTrack) pidx[lr] = p;
+    }
+    X[lr_io] = Normalize(m, v);
+    if (kTrack) pidx[lr_io] = p;
+  }
+  for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
+    const uint32_t row = it * kFRows + lr_io;
+    const bool valid = io_wave && row < n;
+    const uint32_t row_nx = (it + gridDim.x) * kFRows + lr_io;
+    const bool pre = io_wave && it + gridDim.x < n_it && row_nx < n;
+    // next iteration: queue entry (tracker) or input row (batch)
+    uint32_t p_nx = 0;
+    float4 v_nx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (pre) {
+      if (kTrack) p_nx = R.QT[row_nx];
+      else v_nx = reinterpret_cast<const float4*>(x)[row_nx];
+    }
+    // this iteration's finish: the particle's energy, temperature and table
+    uint32_t p = 0;
+    double fE = 0, fT = 0;
+    int fI = 0;
+    if (kTrack && valid) {
+      p = pidx[lr_io];
+      fE = R.B.E[p];
+      fT = R.B.tslT[p];
+      fI = R.B.tslI[p];
     }
     __syncthreads();
     // layer 0 (4 -> 256): thread = (row, 64 consecutive positions)
@@ -263,6 +294,20 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
       }
     }
     __syncthreads();
+    // next iteration (tracker): the queued particle's state; this
+    // iteration's finish: its table's constants
+    double nE = 0, nT = 0;
+    uint32_t ng = 0;
+    if (kTrack && pre) {
+      nE = R.B.E[p_nx];
+      nT = R.B.tslT[p_nx];
+      ng = R.B.rng[p_nx];
+    }
+    double awr = 0, b_cut = 0;
+    if (kTrack && valid) {
+      awr = io.W.tsl[fI].awr;
+      b_cut = io.W.tsl[fI].beta_cutoff;
+    }
     f32x16 acc[kFCb];
     // layer 1 -> H1
     MfmaRows(H0, A1, h, j, acc);
@@ -297,13 +342,10 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     __syncthreads();
     if (!io_wave) continue;
     // finish
-    const int lr = wave * 64 + lane;
-    const uint32_t row = it * kFRows + lr;
-    const bool valid = row < n;
     float o0 = 0.0f, o1 = 0.0f;
 #pragma unroll
     for (int w = 0; w < kMlpWaves; w++) {
-      const float2 s = parts[w * kFRows + lr];
+      const float2 s = parts[w * kFRows + lr_io];
       o0 += s.x;
       o1 += s.y;
     }
@@ -314,25 +356,21 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
         y[size_t(row) * 2 + 0] = y1;
         y[size_t(row) * 2 + 1] = y2;
       }
+      if (pre) X[lr_io] = Normalize(m, v_nx);
       continue;
     }
     // KTslApply of the event loop: the scatter with the network's (alpha,
     // beta); a particle outside the trained box or the kinematic support is
     // parked in the fallback queue for the POD sampler (MlpFlushFallback)
-    const RunState& R = io.R;
     bool alive = false, fall = false;
-    const uint32_t p = pidx[lr];
     if (valid) {
-      double E = R.B.E[p];
-      const double T = R.B.tslT[p];
-      const DTsl t = io.W.tsl[R.B.tslI[p]];
+      const double E = fE, T = fT;
       double a = 0, b = 0;
-      if (InBoxT(io, E, T) &&
-          OutputsToAlphaBeta(y1, y2, E, T, t.awr, t.beta_cutoff, io.e_min, a, b)) {
+      if (InBoxT(io, E, T) && OutputsToAlphaBeta(y1, y2, E, T, awr, b_cut, io.e_min, a, b)) {
         V3 dir = LoadDir(R.B, p);
         uint32_t g = R.B.rng[p];
         const double E_p = E + b * kBoltzmann * T;
-        const double mu = (E + E_p - a * t.awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
+        const double mu = (E + E_p - a * awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
         const double phi = 2 * kPi * Canonical(g);
         dir = Rotate(dir, mu > 1 ? 1.0 : (mu < -1 ? -1.0 : mu), phi);
         if (io.W.n_est_tsl) ScoreAlphaBeta(io.W, R, R.B.hist[p], a, b);
@@ -343,6 +381,13 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
       } else {
         fall = true;
       }
+    }
+    // the next iteration's inputs, from the state loaded above
+    if (pre) {
+      const double u1 = Canonical(ng), u2 = Canonical(ng);
+      R.B.rng[p_nx] = ng;
+      X[lr_io] = Normalize(m, make_float4(float(log(nE)), float(nT), float(u1), float(u2)));
+      pidx[lr_io] = p_nx;
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, io.iter, kQ), alive, tot);
