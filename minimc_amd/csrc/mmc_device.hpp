@@ -139,6 +139,16 @@ struct DevWorld {
   const double* hot;  // thermal-scattering sampler tables (see DTsl)
   const double* tot;  // thermal-scattering total cross-section tables
   int hot_n, tot_n;   // doubles in each
+  // Every descriptor array above (surfaces .. ests) lives in one block,
+  // desc, at the byte offsets desc_off (order of kDesc*), so that a kernel
+  // can copy the whole block into LDS at once (DescInLds).
+  const unsigned char* desc;
+  int desc_bytes;  // a multiple of 16
+  int desc_off[13];
+};
+enum DescArray {
+  kDescSurfaces, kDescCells, kDescFaces, kDescFields, kDescMats, kDescComps, kDescCe,
+  kDescRx, kDescTables, kDescGrids, kDescTsl, kDescParts, kDescEsts, kDescN
 };
 
 // ---- RNG: std::minstd_rand with libstdc++ canonical (see oracle/rng.hpp) --

@@ -239,6 +239,36 @@ __device__ __forceinline__ int HashLdsInts(const DevWorld& W) {
   return (W.energy_mode == 1 && W.grids[0].n_hash > 0) ? ((W.grids[0].n_hash + 2) & ~1) : 0;
 }
 
+// Copies the World's descriptor block into LDS (block-wide; every thread of
+// the block must call it) and returns the World with every descriptor array
+// read from there.  The geometry, material and table descriptors sit on every
+// event's chain of dependent loads (cell -> faces -> surfaces, material ->
+// components -> nuclide -> grid, table, reaction); in LDS each link costs an
+// LDS read instead of a cache round trip.
+__device__ __forceinline__ DevWorld DescInLds(const DevWorld& W, unsigned char* lds) {
+  // 8-byte words: lds is 8-byte aligned (it follows ints padded to an even
+  // count and doubles)
+  const uint2* src = reinterpret_cast<const uint2*>(W.desc);
+  uint2* dst = reinterpret_cast<uint2*>(lds);
+  for (int i = threadIdx.x; i < W.desc_bytes / 8; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  DevWorld L = W;
+  L.surfaces = reinterpret_cast<const DSurface*>(lds + W.desc_off[kDescSurfaces]);
+  L.cells = reinterpret_cast<const DCell*>(lds + W.desc_off[kDescCells]);
+  L.faces = reinterpret_cast<const DFace*>(lds + W.desc_off[kDescFaces]);
+  L.fields = reinterpret_cast<const DField*>(lds + W.desc_off[kDescFields]);
+  L.mats = reinterpret_cast<const DMaterial*>(lds + W.desc_off[kDescMats]);
+  L.comps = reinterpret_cast<const DComp*>(lds + W.desc_off[kDescComps]);
+  L.ce = reinterpret_cast<const DCeNuclide*>(lds + W.desc_off[kDescCe]);
+  L.rx = reinterpret_cast<const DCeReaction*>(lds + W.desc_off[kDescRx]);
+  L.tables = reinterpret_cast<const DTable*>(lds + W.desc_off[kDescTables]);
+  L.grids = reinterpret_cast<const DGrid*>(lds + W.desc_off[kDescGrids]);
+  L.tsl = reinterpret_cast<const DTsl*>(lds + W.desc_off[kDescTsl]);
+  L.parts = reinterpret_cast<const DTslPart*>(lds + W.desc_off[kDescParts]);
+  L.ests = reinterpret_cast<const DEst*>(lds + W.desc_off[kDescEsts]);
+  return L;
+}
+
 // Copies src[0, n) into LDS (block-wide; every thread of the block must call
 // it).
 __device__ __forceinline__ const double* StageHot(const double* src, double* lds, int n) {
@@ -354,17 +384,19 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
 }
 
 #ifdef MMC_TRACKER_KERNELS
-// kLds: the thermal-scattering totals are staged after the hash.
+// kLds: the thermal-scattering totals and the World's descriptors are staged
+// after the hash.
 template <bool kLds>
-__global__ void __launch_bounds__(256) KAdvance(DevWorld W, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
   if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
   const uint32_t n = *Ring(R, iter - 1, kQ);
   if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
-  const int* hash = StageHash(W, lds_hash);
-  const double* Ht =
-      kLds ? StageHot(W.tot, reinterpret_cast<double*>(lds_hash + HashLdsInts(W)), W.tot_n)
-           : W.tot;
+  const int* hash = StageHash(Wg, lds_hash);
+  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
+  const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
+  const DevWorld W =
+      kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
   const uint32_t* Q = R.Q[qin];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
@@ -406,8 +438,13 @@ __device__ __forceinline__ bool CrossEvent(const DevWorld& W, const RunState& R,
 }
 
 #ifdef MMC_TRACKER_KERNELS
-__global__ void __launch_bounds__(256) KCross(DevWorld W, RunState R, int iter, int qin) {
+// kLds: the World's descriptors are staged in LDS.
+template <bool kLds>
+__global__ void __launch_bounds__(256) KCross(DevWorld Wg, RunState R, int iter, int qin) {
+  extern __shared__ uint4 lds_desc[];
   const uint32_t n = *Ring(R, iter, kX);
+  if (blockIdx.x * blockDim.x >= n) return;
+  const DevWorld W = kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_desc)) : Wg;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -710,14 +747,15 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
 
 #ifdef MMC_TRACKER_KERNELS
 template <bool kLds>
-__global__ void __launch_bounds__(256, 2) KCollide(DevWorld W, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int iter, int qin) {
   extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter, kC);
   if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
-  const int* hash = StageHash(W, lds_hash);
-  const double* Ht =
-      kLds ? StageHot(W.tot, reinterpret_cast<double*>(lds_hash + HashLdsInts(W)), W.tot_n)
-           : W.tot;
+  const int* hash = StageHash(Wg, lds_hash);
+  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
+  const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
+  const DevWorld W =
+      kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -780,11 +818,13 @@ __device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R
 // kLds: the hot block is staged in LDS (launched with hot_n doubles of
 // dynamic LDS and only as many blocks as are resident at once).
 template <bool kLds>
-__global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256) KTslPod(DevWorld Wg, RunState R, int iter, int qin) {
   extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter, kT);
   if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
-  const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
+  const double* H = kLds ? StageHot(Wg.hot, lds_hot, Wg.hot_n) : Wg.hot;
+  const DevWorld W =
+      kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_hot + Wg.hot_n)) : Wg;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -855,16 +895,20 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 // POD sampler here.  Used only where a collision cannot start a new particle
 // (k-eigenvalue, or no fission).
 template <bool kLds>
-__global__ void __launch_bounds__(256) KTail(DevWorld W, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCrs = 1, kCol = 2, kTsl = 3, kDone = 4 };
   extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter - 1, kQ);
   if (blockIdx.x * blockDim.x >= n) return;
-  // kLds: the hash, the totals and the sampler's tables in LDS
-  const int* hash = kLds ? StageHash(W, lds_hash) : nullptr;
-  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(W));
-  const double* Ht = kLds ? StageHot(W.tot, lds_tot, W.tot_n) : W.tot;
-  const double* H = kLds ? StageHot(W.hot, lds_tot + W.tot_n, W.hot_n) : W.hot;
+  // kLds: the hash, the totals, the sampler's tables and the World's
+  // descriptors in LDS
+  const int* hash = kLds ? StageHash(Wg, lds_hash) : nullptr;
+  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
+  const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
+  const double* H = kLds ? StageHot(Wg.hot, lds_tot + Wg.tot_n, Wg.hot_n) : Wg.hot;
+  const DevWorld W = kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(
+                                              lds_tot + Wg.tot_n + Wg.hot_n))
+                          : Wg;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   int st = q < n ? kAdv : kDone;

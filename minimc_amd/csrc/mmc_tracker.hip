@@ -32,6 +32,7 @@ static constexpr int kBlock = 256;
 // and the largest total-cross-section part staged by the event kernels.
 static constexpr size_t kHotLdsMax = 128 * 1024;
 static constexpr size_t kTotLdsMax = 32 * 1024;
+static constexpr size_t kDescLdsMax = 24 * 1024;  // the World's descriptor block
 static constexpr size_t kLdsPerBlock = 160 * 1024;
 
 struct mmc_ctx {
@@ -46,11 +47,12 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
-  size_t hot_lds = 0;          // LDS bytes of the hot block (0: KTslPod does not stage it)
+  size_t hot_lds = 0;          // LDS bytes of KTslPod<true> (0: it does not stage)
   uint32_t tsl_grid = 1024;    // grid cap of KTslPod when it stages the hot block
   size_t ev_lds = 0;           // LDS bytes of KAdvance / KCollide
-  bool ev_tot_lds = false;     // ... which stage the hot block's totals after the hash
-  size_t tail_lds = 0;         // LDS bytes of KTail (0: KTail<false>)
+  bool ev_tot_lds = false;     // ... which stage the totals and descriptors after the hash
+  size_t tail_lds = 0;         // LDS bytes of KTail<true> (0: KTail<false>, no LDS)
+  size_t desc_lds = 0;         // LDS bytes of the descriptor block (0: not staged)
   DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
@@ -474,12 +476,6 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   ctx->NB = nb;
   if (F.pool.empty()) F.pool.push_back(0);
   if (F.ipool.empty()) F.ipool.push_back(0);
-  W.surfaces = DevUpload(ctx, surf);
-  W.cells = DevUpload(ctx, cells);
-  W.faces = DevUpload(ctx, faces);
-  W.fields = DevUpload(ctx, fields);
-  W.mats = DevUpload(ctx, mats);
-  W.comps = DevUpload(ctx, comps);
   W.mg_capture = DevUpload(ctx, cap);
   W.mg_scatter_xs = DevUpload(ctx, sxs);
   W.mg_fission = DevUpload(ctx, fis);
@@ -487,12 +483,6 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   W.mg_total = DevUpload(ctx, tot);
   W.mg_scatter_probs = DevUpload(ctx, sp);
   W.mg_chi_probs = DevUpload(ctx, cp);
-  W.ce = DevUpload(ctx, ce);
-  W.rx = DevUpload(ctx, rx);
-  W.tables = DevUpload(ctx, F.tables);
-  W.grids = DevUpload(ctx, F.grids);
-  W.tsl = DevUpload(ctx, tsl);
-  W.parts = DevUpload(ctx, parts);
   W.pool = DevUpload(ctx, F.pool);
   W.ipool = DevUpload(ctx, F.ipool);
   W.hot_n = int(F.hot.size());
@@ -504,17 +494,74 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   // LDS use (layout as HashLdsInts on the device): KTslPod stages the
   // sampler's tables; KAdvance and KCollide the energy hash, then the
   // thermal-scattering totals; KTail the hash, the totals, then the sampler's
-  // tables
+  // tables; each also the descriptor block, below
   ctx->hot_lds = W.hot_n > 0 && size_t(W.hot_n) * sizeof(double) <= kHotLdsMax
                      ? size_t(W.hot_n) * sizeof(double) : 0;
   const size_t hash_lds =
       ctx->n_hash0 > 0 ? sizeof(int) * size_t((ctx->n_hash0 + 2) & ~1) : 0;
   const size_t tot_lds = size_t(W.tot_n) * sizeof(double);
-  ctx->ev_tot_lds = W.tot_n > 0 && tot_lds <= kTotLdsMax;
-  ctx->ev_lds = hash_lds + (ctx->ev_tot_lds ? tot_lds : 0);
-  ctx->tail_lds = ctx->hot_lds && hash_lds + tot_lds + ctx->hot_lds <= kLdsPerBlock
-                      ? hash_lds + tot_lds + ctx->hot_lds : 0;
-  W.ests = DevUpload(ctx, ests);
+  {
+    // the descriptor arrays, in one block (DevWorld::desc)
+    std::vector<unsigned char> blk;
+    auto add = [&](int k, const void* p, size_t bytes) {
+      blk.resize((blk.size() + 15) & ~size_t(15));
+      W.desc_off[k] = int(blk.size());
+      if (bytes)
+        blk.insert(blk.end(), static_cast<const unsigned char*>(p),
+                   static_cast<const unsigned char*>(p) + bytes);
+    };
+    auto addv = [&](int k, const auto& v) { add(k, v.data(), v.size() * sizeof(v[0])); };
+    addv(kDescSurfaces, surf);
+    addv(kDescCells, cells);
+    addv(kDescFaces, faces);
+    addv(kDescFields, fields);
+    addv(kDescMats, mats);
+    addv(kDescComps, comps);
+    addv(kDescCe, ce);
+    addv(kDescRx, rx);
+    addv(kDescTables, F.tables);
+    addv(kDescGrids, F.grids);
+    addv(kDescTsl, tsl);
+    addv(kDescParts, parts);
+    addv(kDescEsts, ests);
+    blk.resize(std::max<size_t>(16, (blk.size() + 15) & ~size_t(15)));
+    unsigned char* d = DevUpload(ctx, blk);
+    if (!d) return Fail(ctx, MMC_ERR_HIP, "descriptor upload failed");
+    W.desc = d;
+    W.desc_bytes = int(blk.size());
+    W.surfaces = reinterpret_cast<const DSurface*>(d + W.desc_off[kDescSurfaces]);
+    W.cells = reinterpret_cast<const DCell*>(d + W.desc_off[kDescCells]);
+    W.faces = reinterpret_cast<const DFace*>(d + W.desc_off[kDescFaces]);
+    W.fields = reinterpret_cast<const DField*>(d + W.desc_off[kDescFields]);
+    W.mats = reinterpret_cast<const DMaterial*>(d + W.desc_off[kDescMats]);
+    W.comps = reinterpret_cast<const DComp*>(d + W.desc_off[kDescComps]);
+    W.ce = reinterpret_cast<const DCeNuclide*>(d + W.desc_off[kDescCe]);
+    W.rx = reinterpret_cast<const DCeReaction*>(d + W.desc_off[kDescRx]);
+    W.tables = reinterpret_cast<const DTable*>(d + W.desc_off[kDescTables]);
+    W.grids = reinterpret_cast<const DGrid*>(d + W.desc_off[kDescGrids]);
+    W.tsl = reinterpret_cast<const DTsl*>(d + W.desc_off[kDescTsl]);
+    W.parts = reinterpret_cast<const DTslPart*>(d + W.desc_off[kDescParts]);
+    W.ests = reinterpret_cast<const DEst*>(d + W.desc_off[kDescEsts]);
+  }
+  // The staging kernels put the descriptor block last in their LDS (KAdvance,
+  // KCollide: hash, totals, descriptors; KCross: descriptors; KTslPod:
+  // sampler tables, descriptors; KTail: all of them).  Their LDS variants
+  // need it to fit.
+  const size_t desc_lds = size_t(W.desc_bytes) <= kDescLdsMax ? size_t(W.desc_bytes) : 0;
+  ctx->desc_lds = desc_lds;
+  if (!desc_lds) {
+    ctx->ev_tot_lds = false;
+    ctx->ev_lds = hash_lds;
+    ctx->hot_lds = 0;
+    ctx->tail_lds = 0;
+  } else {
+    ctx->ev_tot_lds = tot_lds <= kTotLdsMax;
+    ctx->ev_lds = ctx->ev_tot_lds ? hash_lds + tot_lds + desc_lds : hash_lds;
+    if (ctx->hot_lds) ctx->hot_lds += desc_lds;
+    const size_t hot = size_t(W.hot_n) * sizeof(double);
+    const size_t need = hash_lds + tot_lds + hot + desc_lds;
+    ctx->tail_lds = hot <= kHotLdsMax && need <= kLdsPerBlock ? need : 0;
+  }
   ctx->S = DevAlloc<double>(ctx, nb);
   ctx->S2 = DevAlloc<double>(ctx, nb);
   ctx->R.cycS = DevAlloc<double>(ctx, nb);
@@ -859,8 +906,12 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
                        ctx->R, iter, qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_CROSS);
-  hipLaunchKernelGGL(KCross, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R, iter,
-                     qin);
+  if (ctx->desc_lds)
+    hipLaunchKernelGGL(KCross<true>, dim3(grid), dim3(kBlock), ctx->desc_lds, ctx->stream,
+                       ctx->W, ctx->R, iter, qin);
+  else
+    hipLaunchKernelGGL(KCross<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
+                       iter, qin);
   ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_COLLIDE);
   if (ctx->ev_tot_lds)
