@@ -193,7 +193,16 @@ typedef struct mmc_world_desc {
   const mmc_mg_nuclide* mg; /* [n_nuclides] when multigroup             */
   const mmc_ce_nuclide* ce; /* [n_nuclides] when continuous             */
   const mmc_tsl* tsl;       /* [n_tsl]                                  */
+  /* Total cross-section perturbations (TotalCrossSectionPerturbation,
+   * src/Perturbation.cpp:44-83): the perturbed nuclide of each, in input
+   * order, at most MMC_MAX_PERTURBATIONS.  Every particle carries one
+   * indirect effect per perturbation (Particle::indirect_effects,
+   * src/Particle.hpp:105-106). */
+  int32_t n_perturbations;
+  int32_t pad_p;
+  const int32_t* perturbation_nuclide;
 } mmc_world_desc;
+#define MMC_MAX_PERTURBATIONS 4
 
 /* ---- estimators (src/Estimator.cpp:142-151, src/Bins.cpp) ---------------- */
 enum mmc_bins_kind {
@@ -219,11 +228,15 @@ enum mmc_estimator_kind {
   MMC_EST_TSL_ALPHA = 2,
   MMC_EST_TSL_BETA = 3
 };
+/* A current estimator with sensitivity = j + 1 is the sensitivity of that
+ * estimator to perturbation j (CurrentTotalCrossSectionSensitivity,
+ * src/Sensitivity.cpp:45-58): same surface and bins, and each score is the
+ * particle's indirect effect times the current's score. */
 typedef struct mmc_estimator {
   int32_t kind;
   int32_t surface;       /* MMC_EST_CURRENT                             */
   int32_t has_direction; /* cosine bins present                         */
-  int32_t pad;
+  int32_t sensitivity;   /* 0, or 1 + perturbation index                */
   double u, v, w;        /* cosine reference direction (unnormalised)   */
   mmc_bins cosine;
   mmc_bins energy;       /* alpha / beta bins for the TSL histograms    */

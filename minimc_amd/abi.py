@@ -117,7 +117,12 @@ class WorldDesc(C.Structure):
                 ("materials", C.POINTER(Material)),
                 ("components", C.POINTER(Component)),
                 ("mg", C.POINTER(MgNuclide)), ("ce", C.POINTER(CeNuclide)),
-                ("tsl", C.POINTER(Tsl))]
+                ("tsl", C.POINTER(Tsl)),
+                ("n_perturbations", C.c_int32), ("pad_p", C.c_int32),
+                ("perturbation_nuclide", C.POINTER(C.c_int32))]
+
+
+MMC_MAX_PERTURBATIONS = 4
 
 
 class Bins(C.Structure):
@@ -127,7 +132,7 @@ class Bins(C.Structure):
 
 class Estimator(C.Structure):
     _fields_ = [("kind", C.c_int32), ("surface", C.c_int32),
-                ("has_direction", C.c_int32), ("pad", C.c_int32),
+                ("has_direction", C.c_int32), ("sensitivity", C.c_int32),
                 ("u", C.c_double), ("v", C.c_double), ("w", C.c_double),
                 ("cosine", Bins), ("energy", Bins),
                 ("mesh_n", C.c_int32 * 3), ("pad2", C.c_int32),

@@ -54,6 +54,7 @@ struct DField {
 struct DMaterial {
   double aden;
   int comp_begin, comp_count;
+  int pert_mask, pad;  // bit j: the material holds perturbation j's nuclide
 };
 struct DComp {
   double afrac;
@@ -106,7 +107,7 @@ struct DBins {
   int bounds_off, n_bounds;
 };
 struct DEst {
-  int kind, surface, has_dir, pad;
+  int kind, surface, has_dir, sens;  // sens: 0, or 1 + perturbation index
   double dx, dy, dz;
   DBins cosine, energy;
   int mesh_n[3], bin_off;  // bin_off: first global tally bin
@@ -145,7 +146,9 @@ struct DevWorld {
   const unsigned char* desc;
   int desc_bytes;  // a multiple of 16
   int desc_off[13];
+  int n_pert, pad_pert;  // total cross-section perturbations (at most kMaxPert)
 };
+constexpr int kMaxPert = 4;
 enum DescArray {
   kDescSurfaces, kDescCells, kDescFaces, kDescFields, kDescMats, kDescComps, kDescCe,
   kDescRx, kDescTables, kDescGrids, kDescTsl, kDescParts, kDescEsts, kDescN

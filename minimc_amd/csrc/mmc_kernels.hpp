@@ -35,6 +35,10 @@ struct DevBank {
   int32_t* tslI;  // its thermal-scattering table
   int32_t* xs_grid;  // last energy-grid search of the particle: grid (-1: none)
   int32_t* xs_gi;    // and upper-bound index (a hint, verified before use)
+  // indirect effect of each perturbation, [perturbation][slot]
+  // (Particle::indirect_effects, Particle.hpp:105-106); null without
+  // perturbations
+  double* ie;
 };
 
 struct RunState {
@@ -91,6 +95,14 @@ __device__ __forceinline__ V3 LoadPos(const DevBank& B, uint32_t p) {
 __device__ __forceinline__ V3 LoadDir(const DevBank& B, uint32_t p) {
   return {B.u[p], B.v[p], B.w[p]};
 }
+// A new particle's indirect effects start at zero (Particle::SetPerturbations,
+// Particle.cpp:71-73).  The reference gives them only to source particles
+// (FixedSource.cpp:64-65); secondaries have none there and abort a
+// sensitivity tally, here they start at zero.
+__device__ __forceinline__ void ClearIndirect(const DevWorld& W, const RunState& R,
+                                              uint32_t slot) {
+  for (int j = 0; j < W.n_pert; j++) R.B.ie[size_t(j) * R.capacity + slot] = 0;
+}
 
 // Score one value into a global tally bin: per-history records for fixed
 // source, cycle sums for k-eigenvalue.
@@ -112,14 +124,18 @@ __device__ __forceinline__ void ScoreBin(const RunState& R, uint32_t hist, int b
 
 // CurrentEstimator::GetScore + ParticleBins::GetIndex
 // (Estimator.cpp:142-151, Bins.cpp:196-204)
-__device__ __forceinline__ void ScoreCurrent(const DevWorld& W, const RunState& R,
+// A sensitivity estimator (sens = j + 1) scores the particle's indirect
+// effect of perturbation j times the current's score of 1
+// (CurrentTotalCrossSectionSensitivity::GetScore, Sensitivity.cpp:55-58).
+__device__ __forceinline__ void ScoreCurrent(const DevWorld& W, const RunState& R, uint32_t p,
                                              uint32_t hist, int surf, V3 d, double E) {
   for (int e = 0; e < W.n_est; e++) {
     const DEst& est = W.ests[e];
     if (est.kind != 0 || est.surface != surf) continue;
     const int ci = est.has_dir ? BinIndex(W, est.cosine, Dot(V3{est.dx, est.dy, est.dz}, d)) : 0;
     const int ei = BinIndex(W, est.energy, E);
-    ScoreBin(R, hist, est.bin_off + est.energy.n_bins * ci + ei, 1.0);
+    const double v = est.sens ? R.B.ie[size_t(est.sens - 1) * R.capacity + p] * 1.0 : 1.0;
+    ScoreBin(R, hist, est.bin_off + est.energy.n_bins * ci + ei, v);
   }
 }
 __device__ __forceinline__ void ScoreFlux(const DevWorld& W, const RunState& R,
@@ -191,6 +207,7 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   R.B.hist[i] = hist;
   R.B.steps[i] = 0;
   R.B.xs_grid[i] = -1;
+  ClearIndirect(W, R, i);
   ok = cell >= 0 && W.cells[cell].material >= 0;  // born in void: leaks at once
   }
   uint32_t tot;
@@ -218,6 +235,7 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
   R.B.hist[i] = i;
   R.B.steps[i] = 0;
   R.B.xs_grid[i] = -1;
+  ClearIndirect(W, R, i);
   R.B.k_col[i] = 0;
   R.B.k_trk[i] = 0;
   R.B.k_abs[i] = 0;
@@ -324,6 +342,20 @@ __device__ __forceinline__ const int* StageHash(const DevWorld& W, int* lds_hash
 struct AdvanceOut {
   bool to_x, to_c, to_q;
 };
+// Particle::Stream -> TotalCrossSectionPerturbation::Stream
+// (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
+// material that holds the perturbed nuclide, with sigma the collision
+// probability density where the particle stands (the microscopic total, or
+// the majorant with delta tracking: TransportMethod.cpp:79-82, :124-127).
+// Kept out of line: problems without perturbations never call it, and their
+// event code keeps its registers.
+__device__ __noinline__ void StreamIndirect(const DevWorld& W, const RunState& R, uint32_t p,
+                                            int mask, double sigma, double d) {
+  const double inc = 1 / sigma - d;
+  for (int j = 0; j < W.n_pert; j++)
+    if (mask >> j & 1) R.B.ie[size_t(j) * R.capacity + p] += inc;
+}
+
 // hash: the energy hash (LDS) or nullptr; Ht: the thermal-scattering totals
 // (DevWorld::tot or its LDS copy).
 __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
@@ -365,6 +397,8 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     o.to_c = real;
     o.to_q = !real;
   }
+  if (__builtin_expect(W.n_pert != 0, 0))
+    StreamIndirect(W, R, p, W.mats[c.material].pert_mask, sigma, d);
   if (R.keig)
     R.B.k_trk[R.B.hist[p]] += d * aden * MaterialNuFission(W, c.material, E, hash, &memo);
   if (memo.grid >= 0) {
@@ -433,7 +467,7 @@ __device__ __forceinline__ bool CrossEvent(const DevWorld& W, const RunState& R,
     return false;
   }
   R.B.cell[p] = cell;
-  if (W.n_est_current) ScoreCurrent(W, R, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
+  if (W.n_est_current) ScoreCurrent(W, R, p, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
   return W.cells[cell].material >= 0;
 }
 
@@ -501,6 +535,7 @@ __device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, 
   R.B.hist[slot] = hist;
   R.B.steps[slot] = 0;
   R.B.xs_grid[slot] = -1;
+  ClearIndirect(W, R, slot);
   if (live_q) live_q[atomicAdd(live_cnt, 1u)] = slot;
 }
 
@@ -865,8 +900,12 @@ __device__ __forceinline__ void MoveParticle(const DevBank& src, uint32_t p, con
   dst.tslT[i] = src.tslT[p];
   dst.tslI[i] = src.tslI[p];
 }
+__device__ __forceinline__ void MoveIndirect(int n_pert, uint32_t cap, const DevBank& src,
+                                             uint32_t p, const DevBank& dst, uint32_t i) {
+  for (int j = 0; j < n_pert; j++) dst.ie[size_t(j) * cap + i] = src.ie[size_t(j) * cap + p];
+}
 __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin,
-                                                uint32_t* fb) {
+                                                uint32_t* fb, int n_pert) {
   const uint32_t n = *Ring(R, iter - 1, kQ);
   const uint32_t nf = fb ? R.misc[kMiscFallback] : 0u;
   uint32_t* Q = R.Q[qin];
@@ -874,9 +913,11 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
        i += gridDim.x * blockDim.x) {
     if (i < n) {
       MoveParticle(R.B, Q[i], dst, i);
+      MoveIndirect(n_pert, R.capacity, R.B, Q[i], dst, i);
       Q[i] = i;
     } else {
       MoveParticle(R.B, fb[i - n], dst, i);
+      MoveIndirect(n_pert, R.capacity, R.B, fb[i - n], dst, i);
       fb[i - n] = i;
     }
   }

@@ -32,6 +32,7 @@ static constexpr int kBlock = 256;
 // and the largest total-cross-section part staged by the event kernels.
 static constexpr size_t kHotLdsMax = 128 * 1024;
 static constexpr size_t kTotLdsMax = 32 * 1024;
+static_assert(kMaxPert == MMC_MAX_PERTURBATIONS, "perturbation limit");
 static constexpr size_t kDescLdsMax = 24 * 1024;  // the World's descriptor block
 static constexpr size_t kLdsPerBlock = 160 * 1024;
 
@@ -259,6 +260,14 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     fields[i] = DField{f.kind, 0, f.c, f.gx, f.gy, f.gz, f.b,
                        f.kind == MMC_FIELD_CONSTANT ? f.c : f.upper};
   }
+  // total cross-section perturbations
+  if (wd->n_perturbations < 0 || wd->n_perturbations > MMC_MAX_PERTURBATIONS ||
+      (wd->n_perturbations > 0 && !wd->perturbation_nuclide))
+    return Fail(ctx, MMC_ERR_ARG, "bad perturbations");
+  for (int j = 0; j < wd->n_perturbations; j++)
+    if (wd->perturbation_nuclide[j] < 0 || wd->perturbation_nuclide[j] >= wd->n_nuclides)
+      return Fail(ctx, MMC_ERR_ARG, "bad perturbation nuclide");
+  W.n_pert = wd->n_perturbations;
   // Material::AssignNuclides normalises atom fractions (Material.cpp:82-92)
   std::vector<DMaterial> mats(wd->n_materials);
   std::vector<DComp> comps;
@@ -266,12 +275,15 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     const auto& m = wd->materials[i];
     double sum = 0;
     for (int k = 0; k < m.comp_count; k++) sum += wd->components[m.comp_begin + k].afrac;
-    mats[i] = DMaterial{m.aden, int(comps.size()), m.comp_count};
+    mats[i] = DMaterial{m.aden, int(comps.size()), m.comp_count, 0, 0};
     for (int k = 0; k < m.comp_count; k++) {
       const auto& c = wd->components[m.comp_begin + k];
       if (c.nuclide < 0 || c.nuclide >= wd->n_nuclides)
         return Fail(ctx, MMC_ERR_ARG, "bad component nuclide");
       comps.push_back(DComp{c.afrac / sum, c.nuclide, 0});
+      // perturbations of this nuclide act in this material (Perturbation.cpp:71-75)
+      for (int j = 0; j < W.n_pert; j++)
+        if (wd->perturbation_nuclide[j] == c.nuclide) mats[i].pert_mask |= 1 << j;
     }
   }
   const int G = W.G;
@@ -437,6 +449,10 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       DEst d{};
       d.kind = e.kind;
       d.surface = e.surface;
+      if (e.sensitivity < 0 || e.sensitivity > W.n_pert ||
+          (e.sensitivity && e.kind != MMC_EST_CURRENT))
+        return Fail(ctx, MMC_ERR_ARG, "bad estimator sensitivity");
+      d.sens = e.sensitivity;
       d.has_dir = e.has_direction ? 1 : 0;
       const double nrm = std::sqrt(e.u * e.u + e.v * e.v + e.w * e.w);
       d.dx = e.u / nrm;
@@ -602,6 +618,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   B.tslI = DevAlloc<int32_t>(ctx, cap);
   B.xs_grid = DevAlloc<int32_t>(ctx, cap);
   B.xs_gi = DevAlloc<int32_t>(ctx, cap);
+  B.ie = ctx->W.n_pert ? DevAlloc<double>(ctx, size_t(cap) * ctx->W.n_pert) : nullptr;
   // the compaction target: the moved fields get their own arrays, the
   // history-indexed k sums are shared
   DevBank& A = ctx->alt;
@@ -622,6 +639,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   A.xs_gi = DevAlloc<int32_t>(ctx, cap);
   A.tslT = DevAlloc<double>(ctx, cap);
   A.tslI = DevAlloc<int32_t>(ctx, cap);
+  A.ie = ctx->W.n_pert ? DevAlloc<double>(ctx, size_t(cap) * ctx->W.n_pert) : nullptr;
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
@@ -978,6 +996,7 @@ static void SwapBanks(mmc_ctx* ctx) {
   std::swap(B.xs_gi, A.xs_gi);
   std::swap(B.tslT, A.tslT);
   std::swap(B.tslI, A.tslI);
+  std::swap(B.ie, A.ie);
 }
 
 // Runs event iterations until no particle is alive.  The live-queue length
@@ -1032,7 +1051,8 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     for (int k = 0; k < K; k++) {
       if (compact_ok && every > 0 && iter > 0 && iter % every == 0) {
         hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
-                           ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr);
+                           ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr,
+                           ctx->W.n_pert);
         SwapBanks(ctx);
       }
       const int rc = LaunchIteration(ctx, iter, qin, grid);
@@ -1060,7 +1080,8 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
     if (compact_ok && every == 0 && 2 * live < in_use) {
       hipLaunchKernelGGL(KCompact, dim3(Blocks(live)), dim3(kBlock), 0, ctx->stream, ctx->R,
-                         ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr);
+                         ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr,
+                         ctx->W.n_pert);
       SwapBanks(ctx);
       in_use = live;
     }

@@ -35,6 +35,12 @@ each active cycle's per-history mean into S and S2, so W is the number of
 active cycles and the "std dev" line is the standard error of the cycle
 means.
 
+A sensitivity (an entry of Problem.estimators with a "parent") is written
+under its parent estimator, after the parent's scores, as
+Sensitivity::to_string (Sensitivity.cpp:25-28) does: its name
+("<estimator>::<perturbation>"), a '=' rule, a blank line, its score block
+and a newline.
+
 Estimators that the reference does not have (flux mesh, alpha and beta
 histograms) are written in the same layout: the alpha or beta bins take the
 energy block, and a mesh writes its shape in place of the bin edges.
@@ -102,10 +108,17 @@ def _bins_block(e):
 
 
 def estimator_to_string(e, S, S2, total_weight):
-    """Estimator::to_string (Estimator.cpp:48-57); no sensitivities."""
+    """Estimator::to_string (Estimator.cpp:48-57), without its sensitivities."""
     name = e.get("name") or ""
     return (name + "\n" + "=" * len(name) + "\n\n" + _bins_block(e) +
             scores_to_string(S, S2, total_weight))
+
+
+def sensitivity_to_string(e, S, S2, total_weight):
+    """Sensitivity::to_string (Sensitivity.cpp:25-28)."""
+    name = e.get("name") or ""
+    return (name + "\n" + "=" * len(name) + "\n\n" +
+            scores_to_string(S, S2, total_weight) + "\n")
 
 
 def to_string(problem, tallies, total_weight):
@@ -113,8 +126,17 @@ def to_string(problem, tallies, total_weight):
     (S, S2) of problem.estimators[i], as Tracker.tallies(i) returns them."""
     if len(tallies) != len(problem.estimators):
         raise ValueError("one (S, S2) pair per estimator")
-    return "".join("\n" + estimator_to_string(e, S, S2, total_weight) + "\n"
-                   for e, (S, S2) in zip(problem.estimators, tallies))
+    ests = problem.estimators
+    out = []
+    for i, (e, (S, S2)) in enumerate(zip(ests, tallies)):
+        if e.get("parent") is not None:
+            continue
+        text = estimator_to_string(e, S, S2, total_weight)
+        for k, c in enumerate(ests):  # Estimator::to_string, Estimator.cpp:53-55
+            if c.get("parent") == i:
+                text += sensitivity_to_string(c, tallies[k][0], tallies[k][1], total_weight)
+        out.append("\n" + text + "\n")
+    return "".join(out)
 
 
 def out_text(problem, tallies, batchsize, total_weight=None):

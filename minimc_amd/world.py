@@ -36,6 +36,8 @@ class Problem:
         self.nuclides = []      # dicts (mg or ce)
         self.tsl = []           # dicts of numpy arrays
         self.estimators = []    # dicts
+        # total cross-section perturbations: dicts name, nuclide (index)
+        self.perturbations = []
         self.source = None      # dict for SourceDesc
         self.histories = 0
         self.seed = 1
@@ -163,6 +165,11 @@ class Problem:
                     t["beta_cutoff"], t["alpha_cutoff"]))
             w.n_tsl = len(tsls)
             w.tsl = arr(abi.Tsl, tsls)
+        if self.perturbations:
+            pn = np.array([p["nuclide"] for p in self.perturbations], dtype=np.int32)
+            keep.append(pn)
+            w.n_perturbations = len(pn)
+            w.perturbation_nuclide = pn.ctypes.data_as(C.POINTER(C.c_int32))
         ests = [self._estimator(e, keep) for e in self.estimators]
         td = abi.TallyDesc(len(ests), 0, arr(abi.Estimator, ests))
         keep.append(w)
@@ -184,6 +191,7 @@ class Problem:
         est = abi.Estimator()
         est.kind = e.get("kind", abi.MMC_EST_CURRENT)
         est.surface = e.get("surface", -1)
+        est.sensitivity = e.get("sensitivity", 0)
         cos = e.get("cosine")
         est.has_direction = 1 if cos else 0
         if cos:
@@ -427,6 +435,23 @@ def load_xml(path):
         pr.inactive = int(pt.get("inactive"))
         pr.active = int(pt.get("active"))
         pr.source = _source_from_xml(pt.find("initialsource"), multigroup)
+    # perturbations (PerturbationSet, Perturbation.cpp:87-118), before the
+    # estimators that name them
+    pert_names = []
+    pert_root = root.find("perturbations")
+    if pert_root is not None:
+        for pnode in pert_root:
+            if pnode.tag != "total":
+                raise RuntimeError("unknown perturbation " + pnode.tag)
+            nname = pnode.get("nuclide")
+            if nname not in nuc_names:  # World::FindNuclideByName (World.cpp:63-81)
+                raise RuntimeError('Nuclide "%s" not found. Must be one of: [%s]'
+                                   % (nname, "".join('"%s", ' % n for n in nuc_names)))
+            pr.perturbations.append(dict(name=pnode.get("name"),
+                                         nuclide=nuc_names.index(nname)))
+            pert_names.append(pnode.get("name"))
+    if len(pr.perturbations) > abi.MMC_MAX_PERTURBATIONS:
+        raise RuntimeError("at most %d perturbations" % abi.MMC_MAX_PERTURBATIONS)
     est_root = root.find("estimators")
     if est_root is not None:
         for e in est_root:
@@ -443,6 +468,20 @@ def load_xml(path):
                                        float(cos.get("w"))),
                             bins=_bins_from_xml(cos))
                 pr.estimators.append(est)
+                # sensitivities of this current (Estimator.cpp:83-90); each is
+                # named "<estimator>::<perturbation>" (Scorable.cpp:22-28)
+                sens = e.find("sensitivities")
+                if sens is not None:
+                    parent = len(pr.estimators) - 1
+                    for sp in sens:
+                        pname = sp.get("name")
+                        if pname not in pert_names:  # Perturbation.cpp:99-118
+                            raise RuntimeError(
+                                'Perturbation "%s" not found. Must be one of: [%s]'
+                                % (pname, "".join('"%s", ' % n for n in pert_names)))
+                        pr.estimators.append(dict(
+                            est, name=est["name"] + "::" + pname,
+                            sensitivity=pert_names.index(pname) + 1, parent=parent))
             elif e.tag == "flux":
                 m = e.find("mesh")
                 pr.estimators.append(dict(

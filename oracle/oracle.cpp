@@ -364,6 +364,7 @@ Bins MakeBins(const mmc_bins& b) {
 struct Estimator {
   int kind;
   int surface;
+  int sens = 0;  // 0, or 1 + perturbation: CurrentTotalCrossSectionSensitivity
   bool has_dir;
   Pt dir;
   Bins cosine, energy;
@@ -386,6 +387,10 @@ struct Particle {
   int event = MMC_EV_BIRTH;
   int surface = -1;
   uint64_t hist = 0;
+  // indirect effect of each total cross-section perturbation
+  // (Particle::indirect_effects, Particle.hpp:105-106); zero for a new
+  // particle, secondaries included (the reference gives secondaries none)
+  double ie[MMC_MAX_PERTURBATIONS] = {0, 0, 0, 0};
   std::vector<Particle> secondaries;
 };
 
@@ -423,6 +428,7 @@ struct World {
   std::vector<CeNuclide> ce;
   std::vector<Tsl> tsl;
   std::vector<Estimator> ests;
+  std::vector<int> pert_nuclide;  // perturbed nuclide of each perturbation
   Acc total;  // merged tallies
   std::string error;
   // k-eigenvalue state
@@ -976,7 +982,9 @@ void ScoreCurrent(Ctx& c, const Particle& p) {
     if (est.kind != MMC_EST_CURRENT || est.surface != p.surface) continue;
     const size_t ci = est.has_dir ? est.cosine.index(Dot(est.dir, p.dir)) : 0;
     const size_t ei = est.energy.index(p.E);
-    c.pending[e][est.energy.size() * ci + ei] += 1.0;
+    // CurrentTotalCrossSectionSensitivity::GetScore (Sensitivity.cpp:55-58)
+    const double score = est.sens ? p.ie[est.sens - 1] * 1.0 : 1.0;
+    c.pending[e][est.energy.size() * ci + ei] += score;
   }
 }
 void ScoreFluxMesh(Ctx& c, const Particle& p, double sigma_t_macro) {
@@ -997,6 +1005,19 @@ void ScoreFluxMesh(Ctx& c, const Particle& p, double sigma_t_macro) {
       idx = idx * est.mesh_n[a] + i;
     }
     if (in) c.pending[e][idx] += 1.0 / sigma_t_macro;
+  }
+}
+
+// TotalCrossSectionPerturbation::Stream (Perturbation.cpp:68-78), called by
+// Particle::Stream (Particle.cpp:46-53) before the move: in a material that
+// holds the perturbed nuclide, 1 / (collision probability density) -
+// distance.  The density is the microscopic total, or the majorant with
+// delta tracking (TransportMethod.cpp:79-82, :124-127).
+void StreamIndirect(const World& w, const Material& m, double sigma, double d, Particle& p) {
+  for (size_t j = 0; j < w.pert_nuclide.size(); j++) {
+    bool holds = false;
+    for (const auto& comp : m.comps) holds = holds || comp.first == w.pert_nuclide[j];
+    if (holds) p.ie[j] += 1 / sigma - d;
   }
 }
 
@@ -1024,6 +1045,7 @@ int Transport(Ctx& c, Particle& p) {
     if (cross) {
       const double d = ns.second + kNudge;
       if (c.keig) c.acc->k_trk += d * m.aden * MaterialNuFission(w, p);
+      StreamIndirect(w, m, sigma, d, p);
       p.pos = p.pos + p.dir * d;  // Particle::Stream (Particle.cpp:46-53)
       p.cell = (w.quirks & MMC_QUIRK_NUDGE_CROSSING)
                    ? FindCell(w, p.pos)
@@ -1042,6 +1064,7 @@ int Transport(Ctx& c, Particle& p) {
       real = Bernoulli(p.rng, tot / sigma);  // pre-flight position (Q6)
     }
     if (c.keig) c.acc->k_trk += d_coll * m.aden * MaterialNuFission(w, p);
+    StreamIndirect(w, m, sigma, d_coll, p);
     p.pos = p.pos + p.dir * d_coll;
     if (!real) {
       p.event = MMC_EV_VIRTUAL;
@@ -1199,6 +1222,12 @@ int orc_create(const mmc_world_desc* wd, const mmc_tally_desc* td,
     }
     w.materials.push_back(mat);
   }
+  if (wd->n_perturbations < 0 || wd->n_perturbations > MMC_MAX_PERTURBATIONS) {
+    delete ow;
+    return MMC_ERR_ARG;  // bad perturbations
+  }
+  for (int j = 0; j < wd->n_perturbations; j++)
+    w.pert_nuclide.push_back(wd->perturbation_nuclide[j]);
   const int G = w.G;
   if (w.energy_mode == MMC_MULTIGROUP) {
     for (int i = 0; i < wd->n_nuclides; i++) {
@@ -1333,6 +1362,7 @@ int orc_create(const mmc_world_desc* wd, const mmc_tally_desc* td,
       Estimator est;
       est.kind = e.kind;
       est.surface = e.surface;
+      est.sens = e.sensitivity;
       est.has_dir = e.has_direction != 0;
       est.dir = Normalized(Pt{e.u, e.v, e.w});
       est.cosine = MakeBins(est.has_dir ? e.cosine : mmc_bins{});

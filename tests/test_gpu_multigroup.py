@@ -213,3 +213,49 @@ def test_out_file_of_the_reference_input(golden, tmp_path):
     path = tmp_path / "point_source_leakage.out"
     output.write_out(str(path), pr, [t.tallies(e) for e in range(n_est)], pr.histories)
     assert path.read_text() == tg
+
+
+def test_sensitivity_point_absorber_closed_form():
+    """Indirect effects on the GPU: every leaking history of the point
+    absorber streams once from the centre, so the sensitivity tally is
+    (1 / sigma - (R + nudge)) times the leakage tally."""
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    from tests.test_oracle_perturbation import NUDGE, absorber_with_sensitivity
+    pr = absorber_with_sensitivity(radius=2.5, sigma=0.4)
+    n = 50000
+    t = Tracker(pr, max_in_flight=1 << 14)
+    t.run_fixed_source(n)
+    S, S2 = t.tallies(0)
+    Ss, S2s = t.tallies(1)
+    ie = 1 / 0.4 - (2.5 + NUDGE)
+    assert S[0] > 0.1 * n * np.exp(-1.0)
+    np.testing.assert_allclose(Ss[0], S[0] * ie, rtol=1e-12)
+    np.testing.assert_allclose(S2s[0], S2[0] * ie * ie, rtol=1e-12)
+
+
+@pytest.mark.parametrize("tracking", [abi.MMC_SURFACE_TRACKING, abi.MMC_CELL_DELTA_TRACKING])
+def test_sensitivity_ce_slab_matches_oracle(tracking):
+    """Total cross-section perturbation of the slab's nuclide, sensitivities
+    of both leakage currents: many streams per history, the tail kernel and
+    compaction on the GPU side."""
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = configs.c2_slab(n_points=60000, n_res=200)
+    pr.tracking = tracking
+    pr.perturbations.append(dict(name="total", nuclide=0))
+    for i in range(2):
+        e = pr.estimators[i]
+        pr.estimators.append(dict(e, name=e["name"] + "::total", sensitivity=1, parent=i))
+    n = 20000
+    o = Oracle(pr)
+    o.run_fixed_source(n, threads=8)
+    t = Tracker(pr, max_in_flight=1 << 13)
+    t.run_fixed_source(n)
+    for e in (2, 3):
+        So, S2o = o.tallies(e)
+        St, S2t = t.tallies(e)
+        # identical streams except after rare libm branch flips
+        assert abs(So.sum() - St.sum()) <= 0.01 * np.abs(So).sum() + 1e-9, e
+        assert abs(S2o.sum() - S2t.sum()) <= 0.02 * S2o.sum() + 1e-9, e
+        assert np.all(np.isfinite(St))
