@@ -43,6 +43,11 @@ struct DSurface {
 };
 struct DCell {
   int material, face_begin, face_count, field;
+  // A box that holds every point the cell's face tests accept, widened by a
+  // small margin (CellBox on the host): the cell searches skip a cell whose
+  // box misses the point without reading its faces.  The margin keeps a
+  // point that sits on a face just across its surface (a crossing) inside.
+  double lo[3], hi[3];
 };
 struct DFace {
   int surface, inside;
@@ -272,9 +277,14 @@ __device__ __forceinline__ bool SurfContains(const DSurface& s, V3 p) {
       return Dot(p, V3{0, 0, 1}) < s.a;
   }
 }
+__device__ __forceinline__ bool InBox(const DCell& c, V3 p) {
+  return p.x >= c.lo[0] && p.x <= c.hi[0] && p.y >= c.lo[1] && p.y <= c.hi[1] &&
+         p.z >= c.lo[2] && p.z <= c.hi[2];
+}
 __device__ __forceinline__ int FindCell(const DevWorld& W, V3 p) {
   for (int c = 0; c < W.n_cells; c++) {
     const DCell cell = W.cells[c];
+    if (!InBox(cell, p)) continue;  // no face test can accept p
     bool in = true;
     for (int k = 0; k < cell.face_count && in; k++) {
       const DFace f = W.faces[cell.face_begin + k];
@@ -290,6 +300,7 @@ __device__ __forceinline__ int FindCellAcross(const DevWorld& W, V3 p, int surf,
                                               int new_inside) {
   for (int c = 0; c < W.n_cells; c++) {
     const DCell cell = W.cells[c];
+    if (!InBox(cell, p)) continue;  // no face test can accept p
     bool in = true;
     for (int k = 0; k < cell.face_count && in; k++) {
       const DFace f = W.faces[cell.face_begin + k];

@@ -852,14 +852,14 @@ __device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R
 #ifdef MMC_TRACKER_KERNELS
 // kLds: the hot block is staged in LDS (launched with hot_n doubles of
 // dynamic LDS and only as many blocks as are resident at once).
+// (The descriptor block stays in HBM here: a sample reads two descriptors,
+// and the sampler's tables alone keep the LDS at two blocks per CU.)
 template <bool kLds>
-__global__ void __launch_bounds__(256) KTslPod(DevWorld Wg, RunState R, int iter, int qin) {
+__global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter, kT);
   if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
-  const double* H = kLds ? StageHot(Wg.hot, lds_hot, Wg.hot_n) : Wg.hot;
-  const DevWorld W =
-      kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_hot + Wg.hot_n)) : Wg;
+  const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;

@@ -245,8 +245,46 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     if (c.material >= wd->n_materials || c.field < 0 || c.field >= wd->n_fields ||
         c.face_begin < 0 || c.face_begin + c.face_count > wd->n_faces)
       return Fail(ctx, MMC_ERR_ARG, "bad cell " + std::to_string(i));
-    cells[i] = DCell{c.material, c.face_begin, c.face_count, c.field};
+    cells[i] = DCell{c.material, c.face_begin, c.face_count, c.field, {}, {}};
     ctx->cell_material.push_back(c.material);
+  }
+  // Bounding box of each cell, from the surfaces' Contains tests
+  // (SurfContains): a plane's inside is coordinate < a, its outside
+  // coordinate >= a; a sphere's inside lies within center +- r; a CylinderX's
+  // inside (quirk Q1: sqrt(y^2 + z^2) < r^2) within |y|, |z| < r^2; an
+  // outside of a sphere or cylinder bounds nothing.  Each bound is widened by
+  // 1e-9 (1 + |bound|), far above the nudge and the rounding of a point that
+  // has just crossed the surface, so a box never rejects a point the exact
+  // tests (or the forced sense of a crossing) accept.
+  for (int i = 0; i < wd->n_cells; i++) {
+    const auto& c = wd->cells[i];
+    double lo[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL}, hi[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+    for (int k = 0; k < c.face_count; k++) {
+      const auto& f = wd->faces[c.face_begin + k];
+      const auto& sf = wd->surfaces[f.surface];
+      const int axis = sf.kind == MMC_PLANEX ? 0 : sf.kind == MMC_PLANEY ? 1
+                       : sf.kind == MMC_PLANEZ ? 2 : -1;
+      if (axis >= 0) {
+        if (f.inside) hi[axis] = std::min(hi[axis], sf.a);
+        else lo[axis] = std::max(lo[axis], sf.a);
+      } else if (f.inside && sf.kind == MMC_SPHERE) {
+        const double ctr[3] = {sf.a, sf.b, sf.c};
+        for (int a = 0; a < 3; a++) {
+          lo[a] = std::max(lo[a], ctr[a] - sf.r);
+          hi[a] = std::min(hi[a], ctr[a] + sf.r);
+        }
+      } else if (f.inside && sf.kind == MMC_CYLINDERX) {
+        const double h = sf.r * sf.r;
+        for (int a = 1; a < 3; a++) {
+          lo[a] = std::max(lo[a], -h);
+          hi[a] = std::min(hi[a], h);
+        }
+      }
+    }
+    for (int a = 0; a < 3; a++) {
+      cells[i].lo[a] = lo[a] == -HUGE_VAL ? lo[a] : lo[a] - 1e-9 * (1 + std::fabs(lo[a]));
+      cells[i].hi[a] = hi[a] == HUGE_VAL ? hi[a] : hi[a] + 1e-9 * (1 + std::fabs(hi[a]));
+    }
   }
   std::vector<DFace> faces(wd->n_faces);
   for (int i = 0; i < wd->n_faces; i++) {
@@ -560,9 +598,9 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     W.ests = reinterpret_cast<const DEst*>(d + W.desc_off[kDescEsts]);
   }
   // The staging kernels put the descriptor block last in their LDS (KAdvance,
-  // KCollide: hash, totals, descriptors; KCross: descriptors; KTslPod:
-  // sampler tables, descriptors; KTail: all of them).  Their LDS variants
-  // need it to fit.
+  // KCollide: hash, totals, descriptors; KCross: descriptors; KTail: all of
+  // them; KTslPod stages only the sampler's tables).  Their LDS variants need
+  // it to fit.
   const size_t desc_lds = size_t(W.desc_bytes) <= kDescLdsMax ? size_t(W.desc_bytes) : 0;
   ctx->desc_lds = desc_lds;
   if (!desc_lds) {
@@ -573,7 +611,6 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   } else {
     ctx->ev_tot_lds = tot_lds <= kTotLdsMax;
     ctx->ev_lds = ctx->ev_tot_lds ? hash_lds + tot_lds + desc_lds : hash_lds;
-    if (ctx->hot_lds) ctx->hot_lds += desc_lds;
     const size_t hot = size_t(W.hot_n) * sizeof(double);
     const size_t need = hash_lds + tot_lds + hot + desc_lds;
     ctx->tail_lds = hot <= kHotLdsMax && need <= kLdsPerBlock ? need : 0;
