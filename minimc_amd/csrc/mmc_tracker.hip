@@ -599,14 +599,13 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   }
   // The staging kernels put the descriptor block last in their LDS (KAdvance,
   // KCollide: hash, totals, descriptors; KCross: descriptors; KTail: all of
-  // them; KTslPod stages only the sampler's tables).  Their LDS variants need
-  // it to fit.
+  // them).  Their LDS variants need it to fit.  KTslPod stages only the
+  // sampler's tables (hot_lds above).
   const size_t desc_lds = size_t(W.desc_bytes) <= kDescLdsMax ? size_t(W.desc_bytes) : 0;
   ctx->desc_lds = desc_lds;
   if (!desc_lds) {
     ctx->ev_tot_lds = false;
     ctx->ev_lds = hash_lds;
-    ctx->hot_lds = 0;
     ctx->tail_lds = 0;
   } else {
     ctx->ev_tot_lds = tot_lds <= kTotLdsMax;
