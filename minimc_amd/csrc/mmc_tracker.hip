@@ -1156,8 +1156,9 @@ extern "C" int mmc_run_fixed_source(mmc_ctx* ctx, const mmc_source_desc* source,
   HIPCHK(hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   ctx->R.keig = 0;
-  const uint32_t chunk_cap = ctx->fixed_fission ? ctx->cap / 2 : ctx->cap;
+  uint32_t chunk_cap = ctx->fixed_fission ? ctx->cap / 2 : ctx->cap;
   uint64_t done = 0;
+  unsigned long long ev_snap[kEvN];
   while (done < n) {
     const uint32_t m = uint32_t(std::min<uint64_t>(n - done, chunk_cap));
     const uint64_t f = first + done;
@@ -1165,9 +1166,24 @@ extern "C" int mmc_run_fixed_source(mmc_ctx* ctx, const mmc_source_desc* source,
     ctx->R.chunk_first = uint32_t(f);
     int rc = ResetRun(ctx);
     if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ev_snap, ctx->R.ev, sizeof(ev_snap), hipMemcpyDeviceToHost,
+                          ctx->stream));
     hipLaunchKernelGGL(KSourceFixed, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                        ctx->stream, ctx->W, ctx->R, *source, seed0, uint32_t(f), m);
     rc = RunEvents(ctx, m);
+    if (rc == MMC_ERR_OVERFLOW && m > 1) {
+      // The chunk's secondaries or score records did not fit in the bank:
+      // restore the event counters and run the same histories again in
+      // chunks half as large.  A history's random numbers depend only on its
+      // index, and its scores are committed only when its chunk completes,
+      // so the results are those of an uninterrupted run.
+      HIPCHK(hipMemcpyAsync(ctx->R.ev, ev_snap, sizeof(ev_snap), hipMemcpyHostToDevice,
+                            ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->err.clear();
+      chunk_cap = m / 2;
+      continue;
+    }
     if (rc) return rc;
     // commit this chunk's per-history scores
     uint32_t misc[16];
