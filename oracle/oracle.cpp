@@ -32,6 +32,15 @@
 //                  stub): k_inf closed form in an infinite medium; Shannon
 //                  entropy recomputed from the bank in the test.  The
 //                  systematic resampling is parity unpinned.
+//   perturbations  indirect effects (src/Perturbation.cpp:68-78): in a pure
+//                  absorber around a point source every leaking history
+//                  streams once, so its indirect effect is exactly
+//                  1/sigma - (R + nudge) and the sensitivity tally equals
+//                  that times the leakage tally.
+//   output text    orc_format_* write the .out blocks with libstdc++'s own
+//                  std::scientific streams, the formatter the reference
+//                  uses; nothing else pins them (the reference ships no
+//                  .out files).
 #include "oracle.h"
 
 #include <algorithm>
