@@ -44,8 +44,10 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 #            = 32) + surface coefficients (8 per plane, 32 per quadric)
 #   collide: read pos 24 + dir 24 + E 8 + rng 4 + cell 4 + hist 4 + slot 4,
 #            write dir 24 + E 8 + rng 4 + slot 4 = 112 + tables
-#   cross:   read pos 24 + dir 24 + E 8 + surf 4 + cell 4 + hist 4 + slot 4,
-#            write cell 4 + slot 4 + tally record 16 = 96
+#   cross:   a surface crossing runs inside the advance kernel on the
+#            particle's registers: write cell 4, + read hist 4 + tally
+#            record 16 where the surface has current estimators (c2) = 24
+#            per crossing, added to the advance kernel's bytes
 #   tsl:     read dir 24 + E 8 + cell 4 + rng 4 + slot 4, write dir 24 + E 8
 #            + rng 4 + slot 4 = 84 + four rank-8 POD reconstructions (two
 #            bracketing CDF rows each for beta and alpha; 8 coefficients +
@@ -54,10 +56,10 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 BYTES = {
     "c2": {"advance": 108 + 32 + 2 * 8,        # total xs + 2 planes
            "collide": 112 + 3 * 32,            # total, capture, elastic
-           "cross": 96, "tsl": 0},
+           "cross": 24, "tsl": 0},
     "c5": {"advance": 108 + 32 + 6 * 8,        # total xs + 6 cube faces
            "collide": 112 + 3 * 32,            # total, reaction split, elastic
-           "cross": 96,
+           "cross": 4,                         # no current estimators
            "tsl": 84 + 4 * 128},
 }
 MLP_FLOP_PER_SAMPLE = 2 * (4 * 256 + 256 * 256 + 256 * 256 + 256 * 2)
@@ -132,13 +134,16 @@ def _roofline(cfg, kt, counts):
                 "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
                 "flop_per_sample": MLP_FLOP_PER_SAMPLE,
                 "avg_launch_ms": ms / max(launches, 1)}
-    achieved = counts[dom] * BYTES[cfg][dom] / (ms * 1e-3) / 1e9
+    nbytes = counts[dom] * BYTES[cfg][dom]
+    if dom == "advance":
+        nbytes += counts["cross"] * BYTES[cfg]["cross"]
+    achieved = nbytes / (ms * 1e-3) / 1e9
     traffic = _profiled_traffic(dom, "c5_multizone") if cfg == "c5" else None
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE, "
                             "profiles/c5_multizone_hbm_traffic.md)" if traffic else None,
-            "algorithmic_bytes_per_launch": counts[dom] * BYTES[cfg][dom] / max(launches, 1),
+            "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
             "bytes_per_event": BYTES[cfg][dom], "avg_launch_ms": ms / max(launches, 1)}
 
 

@@ -423,8 +423,11 @@ int mmc_get_counters(mmc_ctx* ctx, mmc_counters* out);
  * by HIP events on the context's stream and its duration accumulated per
  * kernel kind (mmc_kernel_kind).  Off by default. */
 enum mmc_kernel_kind {
-  MMC_K_ADVANCE = 0, /* XS lookup + collision distance + CSG distance */
-  MMC_K_CROSS = 1,   /* surface crossing + cell search + current tally */
+  MMC_K_ADVANCE = 0, /* XS lookup + collision distance + CSG distance,
+                        and the surface crossing that follows (cell
+                        search + current tally)                        */
+  MMC_K_CROSS = 1,   /* reserved: crossings run in the advance kernel,
+                        so this kind stays at 0 launches               */
   MMC_K_COLLIDE = 2, /* nuclide + reaction sampling, collision physics */
   MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling: the
                         POD sampler, or the surrogate's gather + apply */

@@ -142,6 +142,14 @@ struct DevWorld {
   const double* pool;
   const int* ipool;
   const DEst* ests;
+  // Cell locator: a uniform grid of loc_n[0] x loc_n[1] x loc_n[2] voxels
+  // from loc_lo, loc_inv voxels per cm (loc_n[0] == 0: no grid).  Voxel v
+  // lists, in cell order, the cells loc_cells[loc_off[v] .. loc_off[v+1])
+  // whose boxes reach into it (widened by a small tolerance).
+  const int* loc_off;
+  const int* loc_cells;
+  double loc_lo[3], loc_inv[3];
+  int loc_n[3], pad_loc;
   const double* hot;  // thermal-scattering sampler tables (see DTsl)
   const double* tot;  // thermal-scattering total cross-section tables
   int hot_n, tot_n;   // doubles in each
@@ -150,14 +158,16 @@ struct DevWorld {
   // can copy the whole block into LDS at once (DescInLds).
   const unsigned char* desc;
   int desc_bytes;  // a multiple of 16
-  int desc_off[13];
+  int desc_off[15];
   int n_pert, pad_pert;  // total cross-section perturbations (at most kMaxPert)
 };
 constexpr int kMaxPert = 4;
 enum DescArray {
   kDescSurfaces, kDescCells, kDescFaces, kDescFields, kDescMats, kDescComps, kDescCe,
-  kDescRx, kDescTables, kDescGrids, kDescTsl, kDescParts, kDescEsts, kDescN
+  kDescRx, kDescTables, kDescGrids, kDescTsl, kDescParts, kDescEsts, kDescLocOff,
+  kDescLocCells, kDescN
 };
+static_assert(kDescN == sizeof(DevWorld::desc_off) / sizeof(int), "one offset per array");
 
 // ---- RNG: std::minstd_rand with libstdc++ canonical (see oracle/rng.hpp) --
 __device__ __forceinline__ uint32_t LcgNext(uint32_t& x) {
@@ -281,8 +291,34 @@ __device__ __forceinline__ bool InBox(const DCell& c, V3 p) {
   return p.x >= c.lo[0] && p.x <= c.hi[0] && p.y >= c.lo[1] && p.y <= c.hi[1] &&
          p.z >= c.lo[2] && p.z <= c.hi[2];
 }
+// The cells a search must test for point p, in cell order: the list of p's
+// voxel in the cell locator ([*b, *e) of loc_cells), or every cell (b = 0,
+// e = n_cells, list = false) where there is no grid or p lies outside it.
+// A cell whose face tests accept p has a box that holds p, and that box
+// reaches into p's voxel, so the first accepting cell of the list is the
+// first accepting cell of the World.  A voxel index rounded across a voxel
+// boundary is covered by the tolerance the lists are built with.
+__device__ __forceinline__ bool LocCandidates(const DevWorld& W, V3 p, int& b, int& e) {
+  b = 0;
+  e = W.n_cells;
+  if (W.loc_n[0] == 0) return false;
+  const double t0 = (p.x - W.loc_lo[0]) * W.loc_inv[0];
+  const double t1 = (p.y - W.loc_lo[1]) * W.loc_inv[1];
+  const double t2 = (p.z - W.loc_lo[2]) * W.loc_inv[2];
+  // NaN fails every comparison and falls back to the full search
+  if (!(t0 >= 0 && t0 < W.loc_n[0] && t1 >= 0 && t1 < W.loc_n[1] && t2 >= 0 &&
+        t2 < W.loc_n[2]))
+    return false;
+  const int v = (int(t0) * W.loc_n[1] + int(t1)) * W.loc_n[2] + int(t2);
+  b = W.loc_off[v];
+  e = W.loc_off[v + 1];
+  return true;
+}
 __device__ __forceinline__ int FindCell(const DevWorld& W, V3 p) {
-  for (int c = 0; c < W.n_cells; c++) {
+  int b, e;
+  const bool list = LocCandidates(W, p, b, e);
+  for (int k0 = b; k0 < e; k0++) {
+    const int c = list ? W.loc_cells[k0] : k0;
     const DCell cell = W.cells[c];
     if (!InBox(cell, p)) continue;  // no face test can accept p
     bool in = true;
@@ -295,10 +331,17 @@ __device__ __forceinline__ int FindCell(const DevWorld& W, V3 p) {
   return -1;
 }
 // Crossing without the nudge quirk: the crossed surface's sense is flipped
-// explicitly, every other face is tested on the point.
+// explicitly, every other face is tested on the point.  kLocate: search the
+// locator's candidates; else every cell in order, a loop whose cell index is
+// the same in every lane (faster for the tail kernel, whose single wave per
+// SIMD waits on each dependent load).
+template <bool kLocate>
 __device__ __forceinline__ int FindCellAcross(const DevWorld& W, V3 p, int surf,
                                               int new_inside) {
-  for (int c = 0; c < W.n_cells; c++) {
+  int b = 0, e = W.n_cells;
+  const bool list = kLocate && LocCandidates(W, p, b, e);
+  for (int k0 = b; k0 < e; k0++) {
+    const int c = list ? W.loc_cells[k0] : k0;
     const DCell cell = W.cells[c];
     if (!InBox(cell, p)) continue;  // no face test can accept p
     bool in = true;

@@ -15,7 +15,7 @@ namespace mmc {
 
 constexpr int kRing = 16;        // iteration counter ring
 constexpr int kRingStride = 8;   // counters per iteration
-enum RingSlot : int { kQ = 0, kC = 1, kX = 2, kT = 3 };
+enum RingSlot : int { kQ = 0, kC = 1, kT = 2 };
 constexpr int kMiscFallback = 5;
 // event counters (u64)
 enum Ev : int {
@@ -27,7 +27,6 @@ struct DevBank {
   double *x, *y, *z, *u, *v, *w, *E;
   uint32_t* rng;
   int32_t* cell;
-  int32_t* surf;
   uint32_t* hist;
   uint32_t* steps;
   double *k_col, *k_trk, *k_abs;  // k-eigenvalue sums, indexed by history
@@ -44,7 +43,7 @@ struct DevBank {
 struct RunState {
   DevBank B;
   uint32_t* Q[2];
-  uint32_t *QC, *QX, *QT;
+  uint32_t *QC, *QT;
   uint32_t* ring;      // kRing * kRingStride
   uint32_t* misc;      // [0] err bits, [1] records, [2] sites, [3] slots,
                        // [kMiscFallback] parked surrogate fallbacks
@@ -203,7 +202,6 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   R.B.E[i] = E;
   R.B.rng[i] = seed;
   R.B.cell[i] = cell;
-  R.B.surf[i] = -1;
   R.B.hist[i] = hist;
   R.B.steps[i] = 0;
   R.B.xs_grid[i] = -1;
@@ -231,7 +229,6 @@ __global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32
   R.B.E[i] = s.E;
   R.B.rng[i] = LcgSeed(s.seed);
   R.B.cell[i] = cell;
-  R.B.surf[i] = -1;
   R.B.hist[i] = i;
   R.B.steps[i] = 0;
   R.B.xs_grid[i] = -1;
@@ -284,6 +281,8 @@ __device__ __forceinline__ DevWorld DescInLds(const DevWorld& W, unsigned char* 
   L.tsl = reinterpret_cast<const DTsl*>(lds + W.desc_off[kDescTsl]);
   L.parts = reinterpret_cast<const DTslPart*>(lds + W.desc_off[kDescParts]);
   L.ests = reinterpret_cast<const DEst*>(lds + W.desc_off[kDescEsts]);
+  L.loc_off = reinterpret_cast<const int*>(lds + W.desc_off[kDescLocOff]);
+  L.loc_cells = reinterpret_cast<const int*>(lds + W.desc_off[kDescLocCells]);
   return L;
 }
 
@@ -334,13 +333,34 @@ __device__ __forceinline__ const int* StageHash(const DevWorld& W, int* lds_hash
   return lds_hash;
 }
 
+// ---- surface crossing (TransportMethod.cpp:67-73, :98-105) ----------------
+// Particle p, streamed to `pos` across surface `surf` out of cell `from`,
+// enters the cell on the other side and scores the current estimators of
+// the surface.  Returns whether it is still alive (surface_cross vs leak).
+template <bool kLocate>
+__device__ __forceinline__ bool CrossAt(const DevWorld& W, const RunState& R, uint32_t p,
+                                        V3 pos, int surf, int from, V3 dir, double E) {
+  const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
+                       ? FindCell(W, pos)
+                       : FindCellAcross<kLocate>(W, pos, surf, 1 - FaceInside(W, from, surf));
+  if (cell < 0) {
+    atomicOr(R.misc, kErrLost);
+    return false;
+  }
+  R.B.cell[p] = cell;
+  if (W.n_est_current) ScoreCurrent(W, R, p, R.B.hist[p], surf, dir, E);
+  return W.cells[cell].material >= 0;
+}
+
 // ---- advance: XS lookup + distance to collision + CSG distance ------------
-// SurfaceTracking/CellDeltaTracking::Transport loop body up to the branch
-// (TransportMethod.cpp:57-66, :93-118).  Next event of particle p: a surface
-// crossing (to_x), a real collision (to_c) or, with delta tracking, a
-// virtual collision (to_q: back to the live queue).
+// SurfaceTracking/CellDeltaTracking::Transport loop body
+// (TransportMethod.cpp:57-73, :93-118).  Next event of particle p: a surface
+// crossing (to_x), done here at once from the registers that hold the
+// particle (x_alive: it entered a cell with a material), a real collision
+// (to_c) or, with delta tracking, a virtual collision (to_q: back to the
+// live queue).
 struct AdvanceOut {
-  bool to_x, to_c, to_q;
+  bool to_x, to_c, to_q, x_alive;
 };
 // Particle::Stream -> TotalCrossSectionPerturbation::Stream
 // (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
@@ -357,11 +377,13 @@ __device__ __noinline__ void StreamIndirect(const DevWorld& W, const RunState& R
 }
 
 // hash: the energy hash (LDS) or nullptr; Ht: the thermal-scattering totals
-// (DevWorld::tot or its LDS copy).
+// (DevWorld::tot or its LDS copy); kLocate: the crossing's cell search uses
+// the cell locator (FindCellAcross).
+template <bool kLocate>
 __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
                                                    uint32_t p, const int* hash,
                                                    const double* Ht) {
-  AdvanceOut o{false, false, false};
+  AdvanceOut o{false, false, false, false};
   const bool delta = W.tracking == 1;
   V3 pos = LoadPos(R.B, p);
   const V3 dir = LoadDir(R.B, p);
@@ -384,7 +406,6 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   double d;
   if (cross) {
     d = d_surf + kNudge;
-    R.B.surf[p] = s;
     o.to_x = true;
   } else {
     bool real = true;
@@ -413,7 +434,9 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   if (st > R.max_steps) {
     atomicOr(R.misc, kErrSteps);
     o.to_x = o.to_c = o.to_q = false;
+    return o;
   }
+  if (o.to_x) o.x_alive = CrossAt<kLocate>(W, R, p, pos, s, cell, dir, E);
   return o;
 }
 
@@ -435,66 +458,23 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
-    AdvanceOut o{false, false, false};
+    AdvanceOut o{false, false, false, false};
     uint32_t p = 0;
     if (valid) {
       p = Q[q];
-      o = AdvanceEvent(W, R, p, hash, Ht);
+      o = AdvanceEvent<true>(W, R, p, hash, Ht);
     }
+    const bool live = o.to_q || o.x_alive;
     uint32_t tot;
-    const uint32_t ix = WavePush(Ring(R, iter, kX), o.to_x, tot);
-    if (o.to_x) R.QX[ix] = p;
     const uint32_t ic = WavePush(Ring(R, iter, kC), o.to_c, tot);
     if (o.to_c) R.QC[ic] = p;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), o.to_q, tot);
-    if (o.to_q) R.Q[qin ^ 1][iq] = p;
+    const uint32_t iq = WavePush(Ring(R, iter, kQ), live, tot);
+    if (live) R.Q[qin ^ 1][iq] = p;
     WaveCount(R.ev + kEvAdvance, valid);
     WaveCount(R.ev + kEvVirtual, o.to_q);
+    WaveCount(R.ev + kEvCross, o.to_x);
   }
 }
-#endif  // MMC_TRACKER_KERNELS
-
-// ---- surface crossing (TransportMethod.cpp:67-73, :98-105) ----------------
-// Returns whether the particle is still alive (surface_cross vs leak).
-__device__ __forceinline__ bool CrossEvent(const DevWorld& W, const RunState& R, uint32_t p) {
-  const V3 pos = LoadPos(R.B, p);
-  const int surf = R.B.surf[p];
-  const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
-                       ? FindCell(W, pos)
-                       : FindCellAcross(W, pos, surf, 1 - FaceInside(W, R.B.cell[p], surf));
-  if (cell < 0) {
-    atomicOr(R.misc, kErrLost);
-    return false;
-  }
-  R.B.cell[p] = cell;
-  if (W.n_est_current) ScoreCurrent(W, R, p, R.B.hist[p], surf, LoadDir(R.B, p), R.B.E[p]);
-  return W.cells[cell].material >= 0;
-}
-
-#ifdef MMC_TRACKER_KERNELS
-// kLds: the World's descriptors are staged in LDS.
-template <bool kLds>
-__global__ void __launch_bounds__(256) KCross(DevWorld Wg, RunState R, int iter, int qin) {
-  extern __shared__ uint4 lds_desc[];
-  const uint32_t n = *Ring(R, iter, kX);
-  if (blockIdx.x * blockDim.x >= n) return;
-  const DevWorld W = kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_desc)) : Wg;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const uint32_t q = base + threadIdx.x;
-    const bool valid = q < n;
-    bool alive = false;
-    uint32_t p = 0;
-    if (valid) {
-      p = R.QX[q];
-      alive = CrossEvent(W, R, p);
-    }
-    uint32_t tot;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
-    if (alive) R.Q[qin ^ 1][iq] = p;
-    WaveCount(R.ev + kEvCross, valid);
-  }
-}
-
 #endif  // MMC_TRACKER_KERNELS
 
 // ---- collisions ------------------------------------------------------------
@@ -531,7 +511,6 @@ __device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, 
   R.B.E[slot] = E;
   R.B.rng[slot] = LcgSeed(seed);
   R.B.cell[slot] = R.B.cell[p];
-  R.B.surf[slot] = -1;
   R.B.hist[slot] = hist;
   R.B.steps[slot] = 0;
   R.B.xs_grid[slot] = -1;
@@ -892,7 +871,6 @@ __device__ __forceinline__ void MoveParticle(const DevBank& src, uint32_t p, con
   dst.E[i] = src.E[p];
   dst.rng[i] = src.rng[p];
   dst.cell[i] = src.cell[p];
-  dst.surf[i] = src.surf[p];
   dst.hist[i] = src.hist[p];
   dst.steps[i] = src.steps[p];
   dst.xs_grid[i] = src.xs_grid[p];
@@ -937,7 +915,7 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 // (k-eigenvalue, or no fission).
 template <bool kLds>
 __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, int qin) {
-  enum { kAdv = 0, kCrs = 1, kCol = 2, kTsl = 3, kDone = 4 };
+  enum { kAdv = 0, kCol = 1, kTsl = 2, kDone = 3 };
   extern __shared__ int lds_hash[];
   const uint32_t n = *Ring(R, iter - 1, kQ);
   if (blockIdx.x * blockDim.x >= n) return;
@@ -969,13 +947,11 @@ __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, 
     if (pick < 0) break;  // every lane of the wave is done
     if (st != pick) continue;
     if (st == kAdv) {
-      const AdvanceOut o = AdvanceEvent(W, R, p, hash, Ht);
+      const AdvanceOut o = AdvanceEvent<false>(W, R, p, hash, Ht);
       c_adv++;
       c_virt += o.to_q ? 1 : 0;
-      st = o.to_q ? kAdv : (o.to_x ? kCrs : (o.to_c ? kCol : kDone));
-    } else if (st == kCrs) {
-      c_cross++;
-      st = CrossEvent(W, R, p) ? kAdv : kDone;
+      c_cross += o.to_x ? 1 : 0;
+      st = (o.to_q || o.x_alive) ? kAdv : (o.to_c ? kCol : kDone);
     } else if (st == kCol) {
       c_coll++;
       bool alive, to_t;

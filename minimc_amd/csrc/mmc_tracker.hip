@@ -53,7 +53,6 @@ struct mmc_ctx {
   size_t ev_lds = 0;           // LDS bytes of KAdvance / KCollide
   bool ev_tot_lds = false;     // ... which stage the totals and descriptors after the hash
   size_t tail_lds = 0;         // LDS bytes of KTail<true> (0: KTail<false>, no LDS)
-  size_t desc_lds = 0;         // LDS bytes of the descriptor block (0: not staged)
   DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
@@ -203,6 +202,75 @@ DBins MakeBins(Flat& F, const mmc_bins& b, bool present) {
       d.n_bins = 1;
   }
   return d;
+}
+
+// Cell locator (DevWorld::loc_*): a uniform grid over the finite bounds of
+// the cell boxes.  Along an axis where no box is bounded the grid has one
+// voxel and loc_inv = 0.  Voxel v lists every cell whose box meets v widened
+// by 1e-6 of a voxel on each side, in cell order; the tolerance covers a
+// voxel index computed across a boundary by rounding.  About 8 voxels per
+// cell, at most 4096, halved until off + list fit in `budget` bytes; no grid
+// (loc_n[0] = 0) for a few cells, or when even 8 voxels do not fit.
+void BuildLocator(const std::vector<DCell>& cells, size_t budget, DevWorld& W,
+                  std::vector<int>& off, std::vector<int>& list) {
+  W.loc_n[0] = W.loc_n[1] = W.loc_n[2] = 0;
+  off.assign(1, 0);
+  list.clear();
+  const int nc = int(cells.size());
+  if (nc <= 4) return;
+  double lo[3], hi[3];
+  int active = 0;
+  for (int a = 0; a < 3; a++) {
+    lo[a] = HUGE_VAL;
+    hi[a] = -HUGE_VAL;
+    for (const DCell& c : cells)
+      for (double v : {c.lo[a], c.hi[a]})
+        if (std::isfinite(v)) {
+          lo[a] = std::min(lo[a], v);
+          hi[a] = std::max(hi[a], v);
+        }
+    if (hi[a] > lo[a]) active++;
+  }
+  if (!active) return;
+  for (int target = std::min(4096, 8 * nc); target >= 8; target /= 2) {
+    const int per = std::max(1, int(std::lround(std::pow(double(target), 1.0 / active))));
+    int n[3];
+    double inv[3], w[3];
+    for (int a = 0; a < 3; a++) {
+      const bool on = hi[a] > lo[a];
+      n[a] = on ? std::min(per, 64) : 1;
+      inv[a] = on ? n[a] / (hi[a] - lo[a]) : 0.0;
+      w[a] = on ? (hi[a] - lo[a]) / n[a] : 0.0;
+    }
+    off.assign(1, 0);
+    list.clear();
+    for (int i = 0; i < n[0]; i++)
+      for (int j = 0; j < n[1]; j++)
+        for (int k = 0; k < n[2]; k++) {
+          const int idx[3] = {i, j, k};
+          for (int c = 0; c < nc; c++) {
+            bool meets = true;
+            for (int a = 0; a < 3 && meets; a++) {
+              if (inv[a] == 0) continue;
+              const double vlo = lo[a] + idx[a] * w[a] - 1e-6 * w[a];
+              const double vhi = lo[a] + (idx[a] + 1) * w[a] + 1e-6 * w[a];
+              meets = cells[c].lo[a] <= vhi && cells[c].hi[a] >= vlo;
+            }
+            if (meets) list.push_back(c);
+          }
+          off.push_back(int(list.size()));
+        }
+    if ((off.size() + list.size()) * sizeof(int) + 32 <= budget) {
+      for (int a = 0; a < 3; a++) {
+        W.loc_n[a] = n[a];
+        W.loc_lo[a] = inv[a] == 0 ? 0.0 : lo[a];
+        W.loc_inv[a] = inv[a];
+      }
+      return;
+    }
+  }
+  off.assign(1, 0);
+  list.clear();
 }
 
 }  // namespace
@@ -578,6 +646,13 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     addv(kDescTsl, tsl);
     addv(kDescParts, parts);
     addv(kDescEsts, ests);
+    // the cell locator, within what keeps a block that fits in LDS there
+    const size_t base = (blk.size() + 15) & ~size_t(15);
+    const size_t budget = base + 32 <= kDescLdsMax ? kDescLdsMax - base - 16 : 64 * 1024;
+    std::vector<int> loc_off, loc_cells;
+    BuildLocator(cells, budget, W, loc_off, loc_cells);
+    addv(kDescLocOff, loc_off);
+    addv(kDescLocCells, loc_cells);
     blk.resize(std::max<size_t>(16, (blk.size() + 15) & ~size_t(15)));
     unsigned char* d = DevUpload(ctx, blk);
     if (!d) return Fail(ctx, MMC_ERR_HIP, "descriptor upload failed");
@@ -596,13 +671,13 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     W.tsl = reinterpret_cast<const DTsl*>(d + W.desc_off[kDescTsl]);
     W.parts = reinterpret_cast<const DTslPart*>(d + W.desc_off[kDescParts]);
     W.ests = reinterpret_cast<const DEst*>(d + W.desc_off[kDescEsts]);
+    W.loc_off = reinterpret_cast<const int*>(d + W.desc_off[kDescLocOff]);
+    W.loc_cells = reinterpret_cast<const int*>(d + W.desc_off[kDescLocCells]);
   }
   // The staging kernels put the descriptor block last in their LDS (KAdvance,
-  // KCollide: hash, totals, descriptors; KCross: descriptors; KTail: all of
-  // them).  Their LDS variants need it to fit.  KTslPod stages only the
+  // KCollide: hash, totals, descriptors; KTail: all of them).  Their LDS variants need it to fit.  KTslPod stages only the
   // sampler's tables (hot_lds above).
   const size_t desc_lds = size_t(W.desc_bytes) <= kDescLdsMax ? size_t(W.desc_bytes) : 0;
-  ctx->desc_lds = desc_lds;
   if (!desc_lds) {
     ctx->ev_tot_lds = false;
     ctx->ev_lds = hash_lds;
@@ -644,7 +719,6 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   B.E = DevAlloc<double>(ctx, cap);
   B.rng = DevAlloc<uint32_t>(ctx, cap);
   B.cell = DevAlloc<int32_t>(ctx, cap);
-  B.surf = DevAlloc<int32_t>(ctx, cap);
   B.hist = DevAlloc<uint32_t>(ctx, cap);
   B.steps = DevAlloc<uint32_t>(ctx, cap);
   B.k_col = DevAlloc<double>(ctx, cap);
@@ -668,7 +742,6 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   A.E = DevAlloc<double>(ctx, cap);
   A.rng = DevAlloc<uint32_t>(ctx, cap);
   A.cell = DevAlloc<int32_t>(ctx, cap);
-  A.surf = DevAlloc<int32_t>(ctx, cap);
   A.hist = DevAlloc<uint32_t>(ctx, cap);
   A.steps = DevAlloc<uint32_t>(ctx, cap);
   A.xs_grid = DevAlloc<int32_t>(ctx, cap);
@@ -679,7 +752,6 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
-  R.QX = DevAlloc<uint32_t>(ctx, cap);
   R.QT = DevAlloc<uint32_t>(ctx, cap);
   R.ring = DevAlloc<uint32_t>(ctx, kRing * kRingStride);
   R.misc = DevAlloc<uint32_t>(ctx, 16);
@@ -701,7 +773,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
   if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x || !A.y || !A.z || !A.u ||
-      !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.surf || !A.hist || !A.steps ||
+      !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.hist || !A.steps ||
       !A.xs_grid || !A.xs_gi || !A.tslT || !A.tslI)
     return Fail(ctx, MMC_ERR_HIP, "out of device memory");
   HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
@@ -959,14 +1031,6 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
     hipLaunchKernelGGL(KAdvance<false>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
                        ctx->R, iter, qin);
   ProfEnd(ctx);
-  ProfBegin(ctx, MMC_K_CROSS);
-  if (ctx->desc_lds)
-    hipLaunchKernelGGL(KCross<true>, dim3(grid), dim3(kBlock), ctx->desc_lds, ctx->stream,
-                       ctx->W, ctx->R, iter, qin);
-  else
-    hipLaunchKernelGGL(KCross<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
-                       iter, qin);
-  ProfEnd(ctx);
   ProfBegin(ctx, MMC_K_COLLIDE);
   if (ctx->ev_tot_lds)
     hipLaunchKernelGGL(KCollide<true>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
@@ -1025,7 +1089,6 @@ static void SwapBanks(mmc_ctx* ctx) {
   std::swap(B.E, A.E);
   std::swap(B.rng, A.rng);
   std::swap(B.cell, A.cell);
-  std::swap(B.surf, A.surf);
   std::swap(B.hist, A.hist);
   std::swap(B.steps, A.steps);
   std::swap(B.xs_grid, A.xs_grid);
