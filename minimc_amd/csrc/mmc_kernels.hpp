@@ -420,8 +420,12 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   }
   if (__builtin_expect(W.n_pert != 0, 0))
     StreamIndirect(W, R, p, W.mats[c.material].pert_mask, sigma, d);
-  if (R.keig)
-    R.B.k_trk[R.B.hist[p]] += d * aden * MaterialNuFission(W, c.material, E, hash, &memo);
+  // k-eigenvalue sums: an increment of +0 (no fissile nuclide here) leaves
+  // the non-negative sum as it is, so its read-modify-write is skipped
+  if (R.keig) {
+    const double nuf = MaterialNuFission(W, c.material, E, hash, &memo);
+    if (nuf != 0) R.B.k_trk[R.B.hist[p]] += d * aden * nuf;
+  }
   if (memo.grid >= 0) {
     R.B.xs_grid[p] = memo.grid;
     R.B.xs_gi[p] = memo.gi;
@@ -733,10 +737,14 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   memo.hint_grid = R.B.xs_grid[p];
   memo.hint_gi = R.B.xs_gi[p];
   memo.tot = Ht;
-  if (W.n_est_flux || R.keig) {
+  // the collision estimator of k: +0 increments (no fissile nuclide here) are
+  // skipped, as for the track-length sum, and so is the total they would
+  // divide when no flux estimator needs it either
+  const double nuf = R.keig ? MaterialNuFission(W, c.material, E, hash, &memo) : 0.0;
+  if (W.n_est_flux || nuf != 0) {
     const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo);
     if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
-    if (R.keig) R.B.k_col[R.B.hist[p]] += MaterialNuFission(W, c.material, E, hash, &memo) / tot;
+    if (nuf != 0) R.B.k_col[R.B.hist[p]] += nuf / tot;
   }
   double kabs = 0;
   int tsl = -1;
@@ -744,7 +752,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
                                R.keig ? &kabs : nullptr, &tsl, &memo);
   if (o.children < 0) atomicOr(R.misc, kErrReaction);
   if (R.keig) {
-    R.B.k_abs[R.B.hist[p]] += kabs;
+    if (kabs != 0) R.B.k_abs[R.B.hist[p]] += kabs;
     if (o.event == MMC_EV_FISSION) R.fis_cnt[R.B.hist[p]] = uint32_t(o.children);
   }
   if (o.tsl) {
