@@ -333,21 +333,42 @@ __device__ __forceinline__ const int* StageHash(const DevWorld& W, int* lds_hash
   return lds_hash;
 }
 
+// Copies particle p of bank src to slot i of bank dst (bank compaction).
+__device__ __forceinline__ void MoveParticle(const DevBank& src, uint32_t p, const DevBank& dst,
+                                             uint32_t i) {
+  dst.x[i] = src.x[p]; dst.y[i] = src.y[p]; dst.z[i] = src.z[p];
+  dst.u[i] = src.u[p]; dst.v[i] = src.v[p]; dst.w[i] = src.w[p];
+  dst.E[i] = src.E[p];
+  dst.rng[i] = src.rng[p];
+  dst.cell[i] = src.cell[p];
+  dst.hist[i] = src.hist[p];
+  dst.steps[i] = src.steps[p];
+  dst.xs_grid[i] = src.xs_grid[p];
+  dst.xs_gi[i] = src.xs_gi[p];
+  dst.tslT[i] = src.tslT[p];
+  dst.tslI[i] = src.tslI[p];
+}
+__device__ __forceinline__ void MoveIndirect(int n_pert, uint32_t cap, const DevBank& src,
+                                             uint32_t p, const DevBank& dst, uint32_t i) {
+  for (int j = 0; j < n_pert; j++) dst.ie[size_t(j) * cap + i] = src.ie[size_t(j) * cap + p];
+}
 // ---- surface crossing (TransportMethod.cpp:67-73, :98-105) ----------------
 // Particle p, streamed to `pos` across surface `surf` out of cell `from`,
-// enters the cell on the other side and scores the current estimators of
-// the surface.  Returns whether it is still alive (surface_cross vs leak).
-template <bool kLocate>
+// enters the cell on the other side (`cell`, stored in the bank unless kMove:
+// the caller stores it elsewhere) and scores the current estimators of the
+// surface.  Returns whether it is still alive (surface_cross vs leak).
+template <bool kLocate, bool kMove>
 __device__ __forceinline__ bool CrossAt(const DevWorld& W, const RunState& R, uint32_t p,
-                                        V3 pos, int surf, int from, V3 dir, double E) {
-  const int cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
-                       ? FindCell(W, pos)
-                       : FindCellAcross<kLocate>(W, pos, surf, 1 - FaceInside(W, from, surf));
+                                        V3 pos, int surf, int from, V3 dir, double E,
+                                        int& cell) {
+  cell = (W.quirks & MMC_QUIRK_NUDGE_CROSSING)
+             ? FindCell(W, pos)
+             : FindCellAcross<kLocate>(W, pos, surf, 1 - FaceInside(W, from, surf));
   if (cell < 0) {
     atomicOr(R.misc, kErrLost);
     return false;
   }
-  R.B.cell[p] = cell;
+  if (!kMove) R.B.cell[p] = cell;
   if (W.n_est_current) ScoreCurrent(W, R, p, R.B.hist[p], surf, dir, E);
   return W.cells[cell].material >= 0;
 }
@@ -378,11 +399,15 @@ __device__ __noinline__ void StreamIndirect(const DevWorld& W, const RunState& R
 
 // hash: the energy hash (LDS) or nullptr; Ht: the thermal-scattering totals
 // (DevWorld::tot or its LDS copy); kLocate: the crossing's cell search uses
-// the cell locator (FindCellAcross).
-template <bool kLocate>
+// the cell locator (FindCellAcross).  kMove: a particle that lives on is
+// written to slot i of bank D (every field a later event reads; the pending
+// thermal collision fields are dead here), not back to slot p: the advance
+// kernel compacts the bank as it goes.
+template <bool kLocate, bool kMove>
 __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
                                                    uint32_t p, const int* hash,
-                                                   const double* Ht) {
+                                                   const double* Ht, const DevBank& D,
+                                                   uint32_t i) {
   AdvanceOut o{false, false, false, false};
   const bool delta = W.tracking == 1;
   V3 pos = LoadPos(R.B, p);
@@ -426,54 +451,83 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     const double nuf = MaterialNuFission(W, c.material, E, hash, &memo);
     if (nuf != 0) R.B.k_trk[R.B.hist[p]] += d * aden * nuf;
   }
-  if (memo.grid >= 0) {
+  if (!kMove && memo.grid >= 0) {
     R.B.xs_grid[p] = memo.grid;
     R.B.xs_gi[p] = memo.gi;
   }
   pos = Add(pos, Mul(dir, d));
-  R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
-  R.B.rng[p] = g;
   const uint32_t st = R.B.steps[p] + 1;
-  R.B.steps[p] = st;
+  if (!kMove) {
+    R.B.x[p] = pos.x; R.B.y[p] = pos.y; R.B.z[p] = pos.z;
+    R.B.rng[p] = g;
+    R.B.steps[p] = st;
+  }
   if (st > R.max_steps) {
     atomicOr(R.misc, kErrSteps);
     o.to_x = o.to_c = o.to_q = false;
     return o;
   }
-  if (o.to_x) o.x_alive = CrossAt<kLocate>(W, R, p, pos, s, cell, dir, E);
+  int to_cell = cell;
+  if (o.to_x) o.x_alive = CrossAt<kLocate, kMove>(W, R, p, pos, s, cell, dir, E, to_cell);
+  if (kMove && (o.to_c || o.to_q || o.x_alive)) {
+    D.x[i] = pos.x; D.y[i] = pos.y; D.z[i] = pos.z;
+    D.u[i] = dir.x; D.v[i] = dir.y; D.w[i] = dir.z;
+    D.E[i] = E;
+    D.rng[i] = g;
+    D.cell[i] = to_cell;
+    D.hist[i] = R.B.hist[p];
+    D.steps[i] = st;
+    D.xs_grid[i] = memo.grid >= 0 ? memo.grid : memo.hint_grid;
+    D.xs_gi[i] = memo.grid >= 0 ? memo.gi : memo.hint_gi;
+    MoveIndirect(W.n_pert, R.capacity, R.B, p, D, i);
+  }
   return o;
 }
 
 #ifdef MMC_TRACKER_KERNELS
 // kLds: the thermal-scattering totals and the World's descriptors are staged
-// after the hash.
-template <bool kLds>
-__global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int iter, int qin) {
+// after the hash.  kMove: the kernel also compacts the bank, as KCompact does:
+// the particle at queue position q is written to slot q of bank D (and the
+// queues it joins hold q), and the particles parked for the POD sampler (fb,
+// length misc[kMiscFallback]) move to the slots after the live ones; the host
+// then swaps the banks.
+template <bool kLds, bool kMove>
+__global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int iter, int qin,
+                                                DevBank D, uint32_t* fb) {
   extern __shared__ int lds_hash[];
   if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
   const uint32_t n = *Ring(R, iter - 1, kQ);
-  if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
+  const uint32_t nf = kMove && fb ? R.misc[kMiscFallback] : 0u;
+  if (kMove && blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n + nf;  // slots in use
+  if (blockIdx.x * blockDim.x >= n + nf) return;  // no work: skip the staging too
   const int* hash = StageHash(Wg, lds_hash);
   double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
   const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
   const DevWorld W =
       kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
   const uint32_t* Q = R.Q[qin];
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n + nf;
+       base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
     AdvanceOut o{false, false, false, false};
     uint32_t p = 0;
     if (valid) {
       p = Q[q];
-      o = AdvanceEvent<true>(W, R, p, hash, Ht);
+      o = AdvanceEvent<true, kMove>(W, R, p, hash, Ht, D, q);
+    } else if (kMove && q < n + nf) {
+      const uint32_t f = fb[q - n];
+      MoveParticle(R.B, f, D, q);
+      MoveIndirect(W.n_pert, R.capacity, R.B, f, D, q);
+      fb[q - n] = q;
     }
+    const uint32_t slot = kMove ? q : p;  // the particle's slot after this kernel
     const bool live = o.to_q || o.x_alive;
     uint32_t tot;
     const uint32_t ic = WavePush(Ring(R, iter, kC), o.to_c, tot);
-    if (o.to_c) R.QC[ic] = p;
+    if (o.to_c) R.QC[ic] = slot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), live, tot);
-    if (live) R.Q[qin ^ 1][iq] = p;
+    if (live) R.Q[qin ^ 1][iq] = slot;
     WaveCount(R.ev + kEvAdvance, valid);
     WaveCount(R.ev + kEvVirtual, o.to_q);
     WaveCount(R.ev + kEvCross, o.to_x);
@@ -872,24 +926,6 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
 // banks, so that the following iterations read the particle state with unit
 // stride.  The k-eigenvalue sums are indexed by history and stay where they
 // are.
-__device__ __forceinline__ void MoveParticle(const DevBank& src, uint32_t p, const DevBank& dst,
-                                             uint32_t i) {
-  dst.x[i] = src.x[p]; dst.y[i] = src.y[p]; dst.z[i] = src.z[p];
-  dst.u[i] = src.u[p]; dst.v[i] = src.v[p]; dst.w[i] = src.w[p];
-  dst.E[i] = src.E[p];
-  dst.rng[i] = src.rng[p];
-  dst.cell[i] = src.cell[p];
-  dst.hist[i] = src.hist[p];
-  dst.steps[i] = src.steps[p];
-  dst.xs_grid[i] = src.xs_grid[p];
-  dst.xs_gi[i] = src.xs_gi[p];
-  dst.tslT[i] = src.tslT[p];
-  dst.tslI[i] = src.tslI[p];
-}
-__device__ __forceinline__ void MoveIndirect(int n_pert, uint32_t cap, const DevBank& src,
-                                             uint32_t p, const DevBank& dst, uint32_t i) {
-  for (int j = 0; j < n_pert; j++) dst.ie[size_t(j) * cap + i] = src.ie[size_t(j) * cap + p];
-}
 __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin,
                                                 uint32_t* fb, int n_pert) {
   const uint32_t n = *Ring(R, iter - 1, kQ);
@@ -955,7 +991,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, 
     if (pick < 0) break;  // every lane of the wave is done
     if (st != pick) continue;
     if (st == kAdv) {
-      const AdvanceOut o = AdvanceEvent<false>(W, R, p, hash, Ht);
+      const AdvanceOut o = AdvanceEvent<false, false>(W, R, p, hash, Ht, R.B, p);
       c_adv++;
       c_virt += o.to_q ? 1 : 0;
       c_cross += o.to_x ? 1 : 0;

@@ -811,17 +811,24 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) ==
           hipSuccess && n_cu > 0) {
     const size_t lds = ctx->ev_lds;
-    int occ_a = 0, occ_c = 0;
-    hipError_t ea, ec;
+    int occ_a = 0, occ_c = 0, occ_m = 0;
+    hipError_t ea, ec, em;
     if (ctx->ev_tot_lds) {
-      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<true>, kBlock, lds);
+      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<true, false>, kBlock,
+                                                        lds);
+      em = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, KAdvance<true, true>, kBlock,
+                                                        lds);
       ec = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide<true>, kBlock, lds);
     } else {
-      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<false>, kBlock, lds);
+      ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<false, false>, kBlock,
+                                                        lds);
+      em = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, KAdvance<false, true>, kBlock,
+                                                        lds);
       ec = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide<false>, kBlock, lds);
     }
-    if (ea != hipSuccess || ec != hipSuccess) occ_a = occ_c = 2;
-    ctx->lds_grid = uint32_t(std::max(1, std::min(occ_a, occ_c))) * uint32_t(n_cu);
+    if (ea != hipSuccess || ec != hipSuccess || em != hipSuccess) occ_a = occ_c = occ_m = 2;
+    ctx->lds_grid =
+        uint32_t(std::max(1, std::min(std::min(occ_a, occ_m), occ_c))) * uint32_t(n_cu);
     // the tail kernel's lanes take the next particle when one ends: launch
     // only the blocks that are resident at once
     int occ_t = 0;
@@ -1020,17 +1027,49 @@ static void ProfCollect(mmc_ctx* ctx) {
   ctx->ev_used = 0;
 }
 
-static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
+// Exchanges the particle fields that KCompact moves between the two banks.
+static void SwapBanks(mmc_ctx* ctx) {
+  DevBank& B = ctx->R.B;
+  DevBank& A = ctx->alt;
+  std::swap(B.x, A.x); std::swap(B.y, A.y); std::swap(B.z, A.z);
+  std::swap(B.u, A.u); std::swap(B.v, A.v); std::swap(B.w, A.w);
+  std::swap(B.E, A.E);
+  std::swap(B.rng, A.rng);
+  std::swap(B.cell, A.cell);
+  std::swap(B.hist, A.hist);
+  std::swap(B.steps, A.steps);
+  std::swap(B.xs_grid, A.xs_grid);
+  std::swap(B.xs_gi, A.xs_gi);
+  std::swap(B.tslT, A.tslT);
+  std::swap(B.tslI, A.tslI);
+  std::swap(B.ie, A.ie);
+}
+
+// move: KAdvance compacts the bank into ctx->alt (with the parked fallback
+// particles fb), and the banks are swapped before the rest of the iteration.
+static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid, bool move,
+                           uint32_t* fb) {
   const size_t lds = ctx->ev_lds;
   const uint32_t sgrid = lds ? std::min(grid, ctx->lds_grid) : grid;
   ProfBegin(ctx, MMC_K_ADVANCE);
-  if (ctx->ev_tot_lds)
-    hipLaunchKernelGGL(KAdvance<true>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
-                       ctx->R, iter, qin);
-  else
-    hipLaunchKernelGGL(KAdvance<false>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
-                       ctx->R, iter, qin);
+  const dim3 ag(sgrid), ab(kBlock);
+  if (ctx->ev_tot_lds) {
+    if (move)
+      hipLaunchKernelGGL((KAdvance<true, true>), ag, ab, lds, ctx->stream, ctx->W, ctx->R, iter,
+                         qin, ctx->alt, fb);
+    else
+      hipLaunchKernelGGL((KAdvance<true, false>), ag, ab, lds, ctx->stream, ctx->W, ctx->R,
+                         iter, qin, ctx->R.B, nullptr);
+  } else {
+    if (move)
+      hipLaunchKernelGGL((KAdvance<false, true>), ag, ab, lds, ctx->stream, ctx->W, ctx->R,
+                         iter, qin, ctx->alt, fb);
+    else
+      hipLaunchKernelGGL((KAdvance<false, false>), ag, ab, lds, ctx->stream, ctx->W, ctx->R,
+                         iter, qin, ctx->R.B, nullptr);
+  }
   ProfEnd(ctx);
+  if (move) SwapBanks(ctx);
   ProfBegin(ctx, MMC_K_COLLIDE);
   if (ctx->ev_tot_lds)
     hipLaunchKernelGGL(KCollide<true>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
@@ -1080,23 +1119,6 @@ extern "C" int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* la
   return MMC_OK;
 }
 
-// Exchanges the particle fields that KCompact moves between the two banks.
-static void SwapBanks(mmc_ctx* ctx) {
-  DevBank& B = ctx->R.B;
-  DevBank& A = ctx->alt;
-  std::swap(B.x, A.x); std::swap(B.y, A.y); std::swap(B.z, A.z);
-  std::swap(B.u, A.u); std::swap(B.v, A.v); std::swap(B.w, A.w);
-  std::swap(B.E, A.E);
-  std::swap(B.rng, A.rng);
-  std::swap(B.cell, A.cell);
-  std::swap(B.hist, A.hist);
-  std::swap(B.steps, A.steps);
-  std::swap(B.xs_grid, A.xs_grid);
-  std::swap(B.xs_gi, A.xs_gi);
-  std::swap(B.tslT, A.tslT);
-  std::swap(B.tslI, A.tslI);
-  std::swap(B.ie, A.ie);
-}
 
 // Runs event iterations until no particle is alive.  The live-queue length
 // is read back every `K` iterations; in between, kernels read queue lengths
@@ -1148,13 +1170,16 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     }
     const uint32_t grid = Blocks(bound);
     for (int k = 0; k < K; k++) {
-      if (compact_ok && every > 0 && iter > 0 && iter % every == 0) {
+      // compaction every iteration is done by the advance kernel itself
+      const bool move = compact_ok && every == 1 && iter > 0;
+      if (compact_ok && every > 1 && iter > 0 && iter % every == 0) {
         hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
                            ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr,
                            ctx->W.n_pert);
         SwapBanks(ctx);
       }
-      const int rc = LaunchIteration(ctx, iter, qin, grid);
+      const int rc =
+          LaunchIteration(ctx, iter, qin, grid, move, surrogate ? ctx->mlp.fb : nullptr);
       if (rc) return rc;
       iter++;
       qin ^= 1;

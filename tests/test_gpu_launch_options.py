@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 SCHEDULES = [
     dict(tail_threshold=-1, compact_interval=-1, graph_iters=8),   # events only
     dict(tail_threshold=1 << 30, compact_interval=-1, graph_iters=4),  # tail early
-    dict(tail_threshold=2048, compact_interval=1, graph_iters=8),  # compaction
+    dict(tail_threshold=2048, compact_interval=1, graph_iters=8),  # compaction in KAdvance
+    dict(tail_threshold=2048, compact_interval=2, graph_iters=8),  # KCompact
     dict(),                                                         # defaults
 ]
 KEYS = ("k_analog", "k_collision", "k_track", "k_absorption", "entropy", "n_sites")
