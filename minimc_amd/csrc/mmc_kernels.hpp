@@ -637,11 +637,14 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
   const DCell c = W.cells[cell];
   const DMaterial m = W.mats[c.material];
   const double T = FieldAt(W.fields[c.field], pos);
-  // SampleNuclide
-  const double threshold_n =
-      MaterialTotal(W, c.material, E, T, false, hash, memo) * Canonical(g);
+  // SampleNuclide.  With one nuclide the loop below returns it whatever the
+  // totals are (the fallback is the last nuclide): only the draw is kept.
   int nuc = W.comps[m.comp_begin + m.comp_count - 1].nuclide;
-  {
+  if (m.comp_count == 1) {
+    (void)Canonical(g);
+  } else {
+    const double threshold_n =
+        MaterialTotal(W, c.material, E, T, false, hash, memo) * Canonical(g);
     double acc = 0;
     for (int k = 0; k < m.comp_count; k++) {
       const DComp cp = W.comps[m.comp_begin + k];
@@ -816,9 +819,13 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   } else {
     alive = o.event == MMC_EV_SCATTER || o.event < 0;
   }
-  R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
-  R.B.E[p] = E;
-  R.B.rng[p] = g;
+  // a particle that died is never read again, and a deferred thermal
+  // scattering keeps its direction and energy until the TSL step
+  if (alive) {
+    R.B.u[p] = dir.x; R.B.v[p] = dir.y; R.B.w[p] = dir.z;
+    R.B.E[p] = E;
+  }
+  if (alive || to_t) R.B.rng[p] = g;
 }
 
 #ifdef MMC_TRACKER_KERNELS

@@ -658,16 +658,22 @@ __device__ __forceinline__ bool InBox(const MlpState& m, double E, double T) {
 }
 // The POD sampler for the parked fallback particles; the survivors join the
 // live queue of iteration `iter` (Q[qin], length in ring slot iter - 1).
+// kLds: the sampler's tables are staged in LDS first (as KTslPod<true>): the
+// few parked samples each wait on a chain of bisection probes.
+template <bool kLds>
 __global__ void __launch_bounds__(256) KTslPodFallback(DevWorld W, RunState R, int iter,
                                                         int qin, const uint32_t* fb) {
+  extern __shared__ double lds_hot[];
   const uint32_t n = R.misc[kMiscFallback];
+  if (blockIdx.x * blockDim.x >= n) return;  // no work: skip the staging too
+  const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     bool alive = false;
     uint32_t p = 0;
     if (q < n) {
       p = fb[q];
-      alive = TslPodEvent(W, R, p, W.hot);
+      alive = TslPodEvent(W, R, p, H);
     }
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter - 1, kQ), alive, tot);
@@ -724,9 +730,14 @@ int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int 
 }
 
 int MlpFlushFallback(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
-                     hipStream_t s) {
+                     size_t hot_lds, hipStream_t s) {
   if (!m.ready || !m.fb) return 1;
-  hipLaunchKernelGGL(KTslPodFallback, dim3(64), dim3(256), 0, s, W, R, iter, qin, m.fb);
+  if (hot_lds)
+    hipLaunchKernelGGL(KTslPodFallback<true>, dim3(64), dim3(256), hot_lds, s, W, R, iter, qin,
+                       m.fb);
+  else
+    hipLaunchKernelGGL(KTslPodFallback<false>, dim3(64), dim3(256), 0, s, W, R, iter, qin,
+                       m.fb);
   hipLaunchKernelGGL(KClearFallback, dim3(1), dim3(1), 0, s, R);
   return 0;
 }

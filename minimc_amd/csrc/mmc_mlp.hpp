@@ -48,8 +48,9 @@ int MlpTslStep(MlpState& m, const DevWorld& W, const RunState& R, int iter, int 
                uint32_t grid, hipStream_t stream, void (*mark)(void*, int), void* mark_arg);
 // The POD sampler for the particles the surrogate parked; they rejoin the
 // live queue of iteration `iter` (Q[qin]).  Called before every read of the
-// live count, so that a parked particle is never missed.
+// live count, so that a parked particle is never missed.  hot_lds: bytes of
+// the sampler's tables to stage in LDS (0: read them from HBM).
 int MlpFlushFallback(MlpState& m, const DevWorld& W, const RunState& R, int iter, int qin,
-                     hipStream_t stream);
+                     size_t hot_lds, hipStream_t stream);
 
 }  // namespace mmc

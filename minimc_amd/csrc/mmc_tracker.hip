@@ -1186,7 +1186,8 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     }
     if (surrogate) {  // parked fallback particles rejoin before the count is read
       ProfBegin(ctx, MMC_K_TSL);
-      const int rc = MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->stream);
+      const int rc =
+          MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->hot_lds, ctx->stream);
       ProfEnd(ctx);
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
     }
