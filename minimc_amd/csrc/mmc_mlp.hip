@@ -365,10 +365,12 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     bool alive = false, fall = false;
     if (valid) {
       const double E = fE, T = fT;
+      // the particle's direction and random state load while the network's
+      // outputs are mapped to (alpha, beta)
+      V3 dir = LoadDir(R.B, p);
+      uint32_t g = R.B.rng[p];
       double a = 0, b = 0;
       if (InBoxT(io, E, T) && OutputsToAlphaBeta(y1, y2, E, T, awr, b_cut, io.e_min, a, b)) {
-        V3 dir = LoadDir(R.B, p);
-        uint32_t g = R.B.rng[p];
         const double E_p = E + b * kBoltzmann * T;
         const double mu = (E + E_p - a * awr * kBoltzmann * T) / (2 * sqrt(E * E_p));
         const double phi = 2 * kPi * Canonical(g);
