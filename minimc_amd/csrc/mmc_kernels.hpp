@@ -80,10 +80,16 @@ __device__ __forceinline__ uint32_t WavePush(uint32_t* counter, bool pred,
   base = __shfl(base, 0);
   return base + prefix;
 }
-__device__ __forceinline__ void WaveCount(unsigned long long* c, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (__lane_id() == 0 && mask) atomicAdd(c, (unsigned long long)__popcll(mask));
-}
+// A wave's event count over all its grid-stride passes, added to the device
+// counter once when the wave is done: one atomic per wave and counter, not
+// one per pass (the counters are shared by every wave of the launch).
+struct WaveTally {
+  unsigned long long n = 0;
+  __device__ __forceinline__ void add(bool pred) { n += __popcll(__ballot(pred)); }
+  __device__ __forceinline__ void flush(unsigned long long* c) const {
+    if (__lane_id() == 0 && n) atomicAdd(c, n);
+  }
+};
 __host__ __device__ __forceinline__ uint32_t* Ring(const RunState& R, int iter, int slot) {
   return R.ring + ((iter + kRing) % kRing) * kRingStride + slot;
 }
@@ -213,10 +219,17 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   if (ok) R.Q[0][iq] = i;
 }
 
-// k-eigenvalue: source site j -> particle slot j.
-__global__ void KSourceSites(DevWorld W, RunState R, const mmc_site* src, uint32_t n) {
+__device__ __forceinline__ DevWorld DescInLds(const DevWorld& W, unsigned char* lds);
+
+// k-eigenvalue: source site j -> particle slot j.  kLds: the cell search
+// reads the World's descriptors from an LDS copy.
+template <bool kLds>
+__global__ void KSourceSites(DevWorld Wg, RunState R, const mmc_site* src, uint32_t n) {
+  extern __shared__ uint4 lds_desc[];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) R.misc[3] = n;
+  if (blockIdx.x * blockDim.x >= n) return;  // whole blocks only: the staging syncs
+  const DevWorld W = kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_desc)) : Wg;
   const bool valid = i < n;
   bool ok = false;
   if (valid) {
@@ -506,6 +519,7 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
   const DevWorld W =
       kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
   const uint32_t* Q = R.Q[qin];
+  WaveTally t_adv, t_virt, t_cross;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n + nf;
        base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
@@ -528,10 +542,13 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
     if (o.to_c) R.QC[ic] = slot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), live, tot);
     if (live) R.Q[qin ^ 1][iq] = slot;
-    WaveCount(R.ev + kEvAdvance, valid);
-    WaveCount(R.ev + kEvVirtual, o.to_q);
-    WaveCount(R.ev + kEvCross, o.to_x);
+    t_adv.add(valid);
+    t_virt.add(o.to_q);
+    t_cross.add(o.to_x);
   }
+  t_adv.flush(R.ev + kEvAdvance);
+  t_virt.flush(R.ev + kEvVirtual);
+  t_cross.flush(R.ev + kEvCross);
 }
 #endif  // MMC_TRACKER_KERNELS
 
@@ -839,6 +856,7 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
   const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
   const DevWorld W =
       kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
+  WaveTally t_col;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -853,8 +871,9 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
     if (alive) R.Q[qin ^ 1][iq] = p;
     const uint32_t it = WavePush(Ring(R, iter, kT), to_t, tot);
     if (to_t) R.QT[it] = p;
-    WaveCount(R.ev + kEvCollide, valid);
+    t_col.add(valid);
   }
+  t_col.flush(R.ev + kEvCollide);
 }
 
 #endif  // MMC_TRACKER_KERNELS
@@ -908,6 +927,7 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
   const uint32_t n = *Ring(R, iter, kT);
   if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
   const double* H = kLds ? StageHot(W.hot, lds_hot, W.hot_n) : W.hot;
+  WaveTally t_tsl;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     const bool valid = q < n;
@@ -920,8 +940,9 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
     uint32_t tot;
     const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
     if (alive) R.Q[qin ^ 1][iq] = p;
-    WaveCount(R.ev + kEvTsl, valid);
+    t_tsl.add(valid);
   }
+  t_tsl.flush(R.ev + kEvTsl);
 }
 
 // ---- bank compaction ----------------------------------------------------------

@@ -250,6 +250,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     X[lr_io] = Normalize(m, v);
     if (kTrack) pidx[lr_io] = p;
   }
+  WaveTally t_tsl, t_fall;  // tracker: the io waves' event counts
   for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
     const uint32_t row = it * kFRows + lr_io;
     const bool valid = io_wave && row < n;
@@ -396,8 +397,12 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     if (alive) R.Q[io.qin ^ 1][iq] = p;
     const uint32_t ifb = WavePush(R.misc + kMiscFallback, fall, tot);
     if (fall) io.fb[ifb] = p;
-    WaveCount(R.ev + kEvTsl, valid);
-    WaveCount(R.ev + kEvFallback, fall);
+    t_tsl.add(valid);
+    t_fall.add(fall);
+  }
+  if (kTrack && io_wave) {
+    t_tsl.flush(io.R.ev + kEvTsl);
+    t_fall.flush(io.R.ev + kEvFallback);
   }
 }
 

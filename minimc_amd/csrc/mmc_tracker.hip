@@ -1471,8 +1471,13 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   int rc = ResetRun(ctx);
   if (rc) return rc;
   if (ctx->NB) FillF64(ctx, R.cycS, 0.0, ctx->NB);
-  hipLaunchKernelGGL(KSourceSites, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0, ctx->stream,
-                     ctx->W, R, ctx->src, N);
+  const size_t src_lds = size_t(ctx->W.desc_bytes) <= kDescLdsMax ? size_t(ctx->W.desc_bytes) : 0;
+  if (src_lds)
+    hipLaunchKernelGGL(KSourceSites<true>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), src_lds,
+                       ctx->stream, ctx->W, R, ctx->src, N);
+  else
+    hipLaunchKernelGGL(KSourceSites<false>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0,
+                       ctx->stream, ctx->W, R, ctx->src, N);
   rc = RunEvents(ctx, N);
   if (rc) return rc;
   uint32_t misc[16];
