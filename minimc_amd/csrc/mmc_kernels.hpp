@@ -80,6 +80,23 @@ __device__ __forceinline__ uint32_t WavePush(uint32_t* counter, bool pred,
   base = __shfl(base, 0);
   return base + prefix;
 }
+// Two pushes of one wave with one atomic: counter pair c[0] (pred0) and
+// c[1] (pred1), adjacent u32s at an 8-byte aligned address, updated as one
+// u64 (low word c[0]; no count comes near 2^32, so no carry crosses).
+__device__ __forceinline__ void WavePush2(uint32_t* c, bool pred0, bool pred1, uint32_t& i0,
+                                          uint32_t& i1) {
+  const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1);
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint64_t base = 0;
+  if (lane == 0 && (m0 | m1))
+    base = atomicAdd(reinterpret_cast<unsigned long long*>(c),
+                     (uint64_t(__popcll(m1)) << 32) | uint64_t(__popcll(m0)));
+  const uint32_t b0 = __shfl(uint32_t(base), 0), b1 = __shfl(uint32_t(base >> 32), 0);
+  i0 = b0 + __popcll(m0 & below);
+  i1 = b1 + __popcll(m1 & below);
+}
+static_assert(kQ % 2 == 0 && kC == kQ + 1 && kRingStride % 2 == 0, "kQ, kC: one u64");
 // A wave's event count over all its grid-stride passes, added to the device
 // counter once when the wave is done: one atomic per wave and counter, not
 // one per pass (the counters are shared by every wave of the launch).
@@ -537,10 +554,9 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
     }
     const uint32_t slot = kMove ? q : p;  // the particle's slot after this kernel
     const bool live = o.to_q || o.x_alive;
-    uint32_t tot;
-    const uint32_t ic = WavePush(Ring(R, iter, kC), o.to_c, tot);
+    uint32_t iq, ic;
+    WavePush2(Ring(R, iter, kQ), live, o.to_c, iq, ic);
     if (o.to_c) R.QC[ic] = slot;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), live, tot);
     if (live) R.Q[qin ^ 1][iq] = slot;
     t_adv.add(valid);
     t_virt.add(o.to_q);
