@@ -15,7 +15,10 @@ namespace mmc {
 
 constexpr int kRing = 16;        // iteration counter ring
 constexpr int kRingStride = 8;   // counters per iteration
-enum RingSlot : int { kQ = 0, kC = 1, kT = 2 };
+// kQ counts the live queue Q, kQ2 the collision survivors' live queue Q2
+// (the next iteration's live particles are Q then Q2), kC the collision
+// queue and kT the thermal-scattering queue.
+enum RingSlot : int { kQ = 0, kC = 1, kT = 2, kQ2 = 3 };
 constexpr int kMiscFallback = 5;
 // event counters (u64)
 enum Ev : int {
@@ -43,6 +46,7 @@ struct DevBank {
 struct RunState {
   DevBank B;
   uint32_t* Q[2];
+  uint32_t* Q2[2];  // live queue of the collision survivors (after Q)
   uint32_t *QC, *QT;
   uint32_t* ring;      // kRing * kRingStride
   uint32_t* misc;      // [0] err bits, [1] records, [2] sites, [3] slots,
@@ -97,6 +101,7 @@ __device__ __forceinline__ void WavePush2(uint32_t* c, bool pred0, bool pred1, u
   i1 = b1 + __popcll(m1 & below);
 }
 static_assert(kQ % 2 == 0 && kC == kQ + 1 && kRingStride % 2 == 0, "kQ, kC: one u64");
+static_assert(kT % 2 == 0 && kQ2 == kT + 1, "kT, kQ2: one u64");
 // A wave's event count over all its grid-stride passes, added to the device
 // counter once when the wave is done: one atomic per wave and counter, not
 // one per pass (the counters are shared by every wave of the launch).
@@ -109,6 +114,16 @@ struct WaveTally {
 };
 __host__ __device__ __forceinline__ uint32_t* Ring(const RunState& R, int iter, int slot) {
   return R.ring + ((iter + kRing) % kRing) * kRingStride + slot;
+}
+// Entry q of the live queue made by iteration `iter` (Q[qin] of length
+// n1 = kQ, then Q2[qin]), and its length.
+__device__ __forceinline__ uint32_t LiveLength(const RunState& R, int iter, uint32_t& n1) {
+  n1 = *Ring(R, iter, kQ);
+  return n1 + *Ring(R, iter, kQ2);
+}
+__device__ __forceinline__ uint32_t& LiveAt(const RunState& R, int qin, uint32_t n1,
+                                            uint32_t q) {
+  return q < n1 ? R.Q[qin][q] : R.Q2[qin][q - n1];
 }
 
 __device__ __forceinline__ V3 LoadPos(const DevBank& B, uint32_t p) {
@@ -526,7 +541,8 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
                                                 DevBank D, uint32_t* fb) {
   extern __shared__ int lds_hash[];
   if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
-  const uint32_t n = *Ring(R, iter - 1, kQ);
+  uint32_t n1;
+  const uint32_t n = LiveLength(R, iter - 1, n1);
   const uint32_t nf = kMove && fb ? R.misc[kMiscFallback] : 0u;
   if (kMove && blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n + nf;  // slots in use
   if (blockIdx.x * blockDim.x >= n + nf) return;  // no work: skip the staging too
@@ -535,7 +551,6 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
   const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
   const DevWorld W =
       kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n)) : Wg;
-  const uint32_t* Q = R.Q[qin];
   WaveTally t_adv, t_virt, t_cross;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n + nf;
        base += gridDim.x * blockDim.x) {
@@ -544,7 +559,7 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
     AdvanceOut o{false, false, false, false};
     uint32_t p = 0;
     if (valid) {
-      p = Q[q];
+      p = LiveAt(R, qin, n1, q);
       o = AdvanceEvent<true, kMove>(W, R, p, hash, Ht, D, q);
     } else if (kMove && q < n + nf) {
       const uint32_t f = fb[q - n];
@@ -882,11 +897,10 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
       p = R.QC[q];
       CollideEvent(W, R, p, hash, Ht, R.Q[qin ^ 1], Ring(R, iter, kQ), alive, to_t);
     }
-    uint32_t tot;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
-    if (alive) R.Q[qin ^ 1][iq] = p;
-    const uint32_t it = WavePush(Ring(R, iter, kT), to_t, tot);
+    uint32_t it, iq;
+    WavePush2(Ring(R, iter, kT), to_t, alive, it, iq);
     if (to_t) R.QT[it] = p;
+    if (alive) R.Q2[qin ^ 1][iq] = p;
     t_col.add(valid);
   }
   t_col.flush(R.ev + kEvCollide);
@@ -972,15 +986,16 @@ __global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter,
 // are.
 __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int iter, int qin,
                                                 uint32_t* fb, int n_pert) {
-  const uint32_t n = *Ring(R, iter - 1, kQ);
+  uint32_t n1;
+  const uint32_t n = LiveLength(R, iter - 1, n1);
   const uint32_t nf = fb ? R.misc[kMiscFallback] : 0u;
-  uint32_t* Q = R.Q[qin];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n + nf;
        i += gridDim.x * blockDim.x) {
     if (i < n) {
-      MoveParticle(R.B, Q[i], dst, i);
-      MoveIndirect(n_pert, R.capacity, R.B, Q[i], dst, i);
-      Q[i] = i;
+      uint32_t& e = LiveAt(R, qin, n1, i);
+      MoveParticle(R.B, e, dst, i);
+      MoveIndirect(n_pert, R.capacity, R.B, e, dst, i);
+      e = i;
     } else {
       MoveParticle(R.B, fb[i - n], dst, i);
       MoveIndirect(n_pert, R.capacity, R.B, fb[i - n], dst, i);
@@ -1005,7 +1020,8 @@ template <bool kLds>
 __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCol = 1, kTsl = 2, kDone = 3 };
   extern __shared__ int lds_hash[];
-  const uint32_t n = *Ring(R, iter - 1, kQ);
+  uint32_t n1;
+  const uint32_t n = LiveLength(R, iter - 1, n1);
   if (blockIdx.x * blockDim.x >= n) return;
   // kLds: the hash, the totals, the sampler's tables and the World's
   // descriptors in LDS
@@ -1019,7 +1035,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, 
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   int st = q < n ? kAdv : kDone;
-  uint32_t p = st == kAdv ? R.Q[qin][q] : 0u;
+  uint32_t p = st == kAdv ? LiveAt(R, qin, n1, q) : 0u;
   unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0;
   unsigned long long c_tail = st == kAdv ? 1 : 0;
   for (;;) {
@@ -1052,7 +1068,7 @@ __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, 
     if (st == kDone) {
       q += stride;
       if (q < n) {
-        p = R.Q[qin][q];
+        p = LiveAt(R, qin, n1, q);
         st = kAdv;
         c_tail++;
       }

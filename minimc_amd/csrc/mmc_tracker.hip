@@ -751,6 +751,8 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   A.ie = ctx->W.n_pert ? DevAlloc<double>(ctx, size_t(cap) * ctx->W.n_pert) : nullptr;
   R.Q[0] = DevAlloc<uint32_t>(ctx, cap);
   R.Q[1] = DevAlloc<uint32_t>(ctx, cap);
+  R.Q2[0] = DevAlloc<uint32_t>(ctx, cap);
+  R.Q2[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
   R.QT = DevAlloc<uint32_t>(ctx, cap);
   R.ring = DevAlloc<uint32_t>(ctx, kRing * kRingStride);
@@ -772,7 +774,8 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->scan_tmp = DevAlloc<uint32_t>(ctx, ctx->scan_cap);
   ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
-  if (!ctx->kpart || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x || !A.y || !A.z || !A.u ||
+  if (!ctx->kpart || !R.Q2[0] || !R.Q2[1] || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x ||
+      !A.y || !A.z || !A.u ||
       !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.hist || !A.steps ||
       !A.xs_grid || !A.xs_gi || !A.tslT || !A.tslI)
     return Fail(ctx, MMC_ERR_HIP, "out of device memory");
@@ -1200,7 +1203,8 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
     ctx->cnt.iterations += K;
     const int rc = CheckMisc(ctx, misc);
     if (rc) return rc;
-    const uint32_t live = ring[((iter - 1 + kRing) % kRing) * kRingStride + kQ];
+    const uint32_t* row = ring.data() + ((iter - 1 + kRing) % kRing) * kRingStride;
+    const uint32_t live = row[kQ] + row[kQ2];  // live queue: Q, then Q2
     if (live == 0) break;
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
     if (compact_ok && every == 0 && 2 * live < in_use) {
