@@ -584,13 +584,17 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
 #endif  // MMC_TRACKER_KERNELS
 
 // ---- collisions ------------------------------------------------------------
+// site: for k-eigenvalue, the fission bank entry the caller claimed for this
+// child (kClaimSite: claim one here).
+constexpr uint32_t kClaimSite = 0xffffffffu;
 __device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, uint32_t p,
                                           uint32_t& g, V3 pos, V3 d, double E, int child,
-                                          uint32_t* live_q, uint32_t* live_cnt) {
+                                          uint32_t* live_q, uint32_t* live_cnt,
+                                          uint32_t site = kClaimSite) {
   const uint32_t seed = LcgNext(g);  // Particle::BankSecondaries (Particle.cpp:96-100)
   const uint32_t hist = R.B.hist[p];
   if (R.keig) {
-    const uint32_t k = atomicAdd(R.misc + 2, 1u);
+    const uint32_t k = site != kClaimSite ? site : atomicAdd(R.misc + 2, 1u);
     if (k >= R.site_cap) {
       atomicOr(R.misc, kErrOverflow);
       return;
@@ -795,10 +799,16 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
       } else {
         out.event = MMC_EV_FISSION;
         const int yield = int(TabAt(W, r.nubar, -1, E) + Canonical(g));
+        // every child is banked: one atomic claims the fission's bank entries
+        // (their order in the bank is fixed later by the site keys)
+        const uint32_t site0 = R && R->keig && yield > 0
+                                   ? atomicAdd(R->misc + 2, uint32_t(yield)) : kClaimSite;
         for (int i = 0; i < yield; i++) {
           const V3 d = IsotropicDirection(g);
           const double Ec = r.chi_kind == 1 ? SampleWatt(g, r.watt_a, r.watt_b) : E;
-          if (R) BankChild(W, *R, p, g, pos, d, Ec, out.children, live_q, live_cnt);
+          if (R)
+            BankChild(W, *R, p, g, pos, d, Ec, out.children, live_q, live_cnt,
+                      R->keig ? site0 + uint32_t(i) : kClaimSite);
           else LcgNext(g);
           out.children++;
         }
