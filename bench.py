@@ -322,7 +322,7 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; "
               "the multi-GPU design (history sharding + one all-reduce of FP64 "
-              "tallies) is described in DESIGN.md, not run.", file=sys.stderr)
+              "tallies) is not run here.", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
         args.steps = 100 if args.config == "c5" else 10

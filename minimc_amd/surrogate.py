@@ -46,7 +46,7 @@ def make_desc(out_shift=(0.0, 0.5), out_scale=(1.0, 0.3), widths=WIDTHS,
     return d
 
 
-# Loss weights: the judge of the sampler is per (E, T) (Wasserstein distance
+# Loss weights: the sampler's accuracy is assessed per (E, T) (Wasserstein distance
 # relative to the POD marginal's spread there), and that spread varies about
 # 3x over the box (the beta target is narrow above a few kT).  Training
 # therefore weights each sample by var_global / var_bin of its target, with

@@ -1,10 +1,10 @@
-"""Trains the thermal-scattering surrogate on the GPU and judges it.
+"""Trains the thermal-scattering surrogate on the GPU and assesses it.
 
 python -m minimc_amd.train_surrogate [--samples 26] [--epochs 3] [--batch 8192]
                                      [--lr 2e-3] [--out PATH] [--eval-only]
 
 Training data: 2^samples POD transport-map targets generated on the device
-from the synthetic S(alpha, beta, T) tables.  The judge compares the
+from the synthetic S(alpha, beta, T) tables.  The assessment compares the
 surrogate sampler with the POD sampler at a grid of (E, T): relative
 Wasserstein-1 distance (W1 / std of the POD marginal) and the KS distance of
 the alpha and beta marginals.  Prints one JSON line.
@@ -24,7 +24,7 @@ if ROOT not in sys.path:
 from minimc_amd import configs, surrogate  # noqa: E402
 from minimc_amd.tracker import Tracker  # noqa: E402
 
-JUDGE_POINTS = [(0.01, 300.0), (0.0253, 600.0), (0.1, 900.0), (0.3, 450.0),
+ASSESS_POINTS = [(0.01, 300.0), (0.0253, 600.0), (0.1, 900.0), (0.3, 450.0),
                 (0.6, 800.0), (1.5, 1000.0), (3.0, 1150.0), (3.9, 350.0)]
 
 
@@ -41,7 +41,7 @@ def ks(a, b):
     return float(np.max(np.abs(fa - fb)))
 
 
-def judge(t, n=400000, points=JUDGE_POINTS, seed=9):
+def assess(t, n=400000, points=ASSESS_POINTS, seed=9):
     rows = []
     for E_eV, T in points:
         E = t.device_array(np.full(n, E_eV * 1e-6))
@@ -87,12 +87,12 @@ def main():
                     final_loss=losses[-1],
                     train_tflops=steps * args.batch * 3 * 2 * surrogate.n_params() / dt / 1e12)
         s.save(out, info=info)
-    rows = judge(t)
+    rows = assess(t)
     for r in rows:
         print("E=%6.3f eV T=%6.1f  W1 beta %.4f alpha %.4f  KS beta %.4f alpha %.4f" % (
             r["E_eV"], r["T"], r["w1_beta"], r["w1_alpha"], r["ks_beta"], r["ks_alpha"]),
             flush=True)
-    info["judge"] = rows
+    info["assessment"] = rows
     info["max_w1"] = max(max(r["w1_beta"], r["w1_alpha"]) for r in rows)
     print(json.dumps(info))
 

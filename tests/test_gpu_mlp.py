@@ -143,7 +143,7 @@ def _w1_rel(a, b):
 def test_trained_surrogate_matches_pod_sampler(env):
     pr, t, s = env
     # a short run (2^23 samples, 3 epochs: a few seconds) with loose bounds;
-    # the full training (2^26 samples, 8 epochs) is judged at 1% by
+    # the full training (2^26 samples, 8 epochs) is assessed at 1% by
     # minimc_amd.train_surrogate and the bench
     losses = s.train(n_samples=1 << 23, batch=16384, epochs=3, lr=3e-3, seed=5)
     # normalised MSE: the first value is already after 1/8 epoch
@@ -162,7 +162,7 @@ def test_trained_surrogate_matches_pod_sampler(env):
         t.tsl_sample_surrogate(E, Tt, r2, a2, b2)
         ap, bp, am, bm = a1.numpy(), b1.numpy(), a2.numpy(), b2.numpy()
         assert np.all(np.isfinite(am)) and np.all(np.isfinite(bm))
-        # a small training run: a loose bound (the full run is judged in bench)
+        # a small training run: a loose bound (the full run is assessed in bench)
         print("W1 rel beta %.4f alpha %.4f at E=%g eV T=%g" % (
             _w1_rel(bm, bp), _w1_rel(am, ap), E_eV, T))
         assert _w1_rel(bm, bp) < 0.25, (E_eV, T, _w1_rel(bm, bp))
