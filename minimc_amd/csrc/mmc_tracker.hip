@@ -43,6 +43,7 @@ struct mmc_ctx {
   mmc_options opt{};
   DevWorld W{};
   std::vector<void*> allocs;
+  uint32_t* host_rb = nullptr;  // pinned copy of the ring and misc (one read per batch)
   bool has_tsl = false;
   bool fixed_fission = false;  // fixed-source problem with fission children
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
@@ -755,8 +756,12 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   R.Q2[1] = DevAlloc<uint32_t>(ctx, cap);
   R.QC = DevAlloc<uint32_t>(ctx, cap);
   R.QT = DevAlloc<uint32_t>(ctx, cap);
-  R.ring = DevAlloc<uint32_t>(ctx, kRing * kRingStride);
-  R.misc = DevAlloc<uint32_t>(ctx, 16);
+  // misc follows the ring, so that one copy reads both
+  R.ring = DevAlloc<uint32_t>(ctx, kRing * kRingStride + 16);
+  R.misc = R.ring ? R.ring + kRing * kRingStride : nullptr;
+  if (hipHostMalloc(reinterpret_cast<void**>(&ctx->host_rb),
+                    sizeof(uint32_t) * (kRing * kRingStride + 16)) != hipSuccess)
+    ctx->host_rb = nullptr;
   R.ev = DevAlloc<unsigned long long>(ctx, kEvN);
   R.capacity = cap;
   R.rec_cap = cap * 4 + 1024;
@@ -774,7 +779,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   ctx->scan_tmp = DevAlloc<uint32_t>(ctx, ctx->scan_cap);
   ctx->kpart_cap = Blocks(cap, 1u << 30) * 3;
   ctx->kpart = DevAlloc<double>(ctx, ctx->kpart_cap);
-  if (!ctx->kpart || !R.Q2[0] || !R.Q2[1] || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x ||
+  if (!ctx->kpart || !ctx->host_rb || !R.Q2[0] || !R.Q2[1] || !B.tslI || !B.xs_grid || !B.xs_gi || !A.x ||
       !A.y || !A.z || !A.u ||
       !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.hist || !A.steps ||
       !A.xs_grid || !A.xs_gi || !A.tslT || !A.tslI)
@@ -862,6 +867,7 @@ extern "C" void mmc_destroy(mmc_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void* p : ctx->allocs) hipFree(p);
+  if (ctx->host_rb) hipHostFree(ctx->host_rb);
   MlpFree(ctx->mlp);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1137,8 +1143,9 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   }
   uint32_t bound = n0;
   int iter = 0, qin = 0;
-  std::vector<uint32_t> ring(kRing * kRingStride);
-  uint32_t misc[16];
+  uint32_t* const ring = ctx->host_rb;  // pinned: the ring, then misc
+  uint32_t* const misc = ctx->host_rb + kRing * kRingStride;
+  constexpr size_t kMiscBytes = 16 * sizeof(uint32_t);
   // history-based tail (KTail) once few particles are left, where no
   // collision can start a new particle
   const bool tail_ok = ctx->opt.tail_threshold >= 0 && (ctx->R.keig || !ctx->fixed_fission);
@@ -1163,7 +1170,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
         hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                            iter, qin);
       ProfEnd(ctx);
-      HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
+      HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, kMiscBytes, hipMemcpyDeviceToHost,
                             ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));
       ProfCollect(ctx);
@@ -1194,16 +1201,15 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       ProfEnd(ctx);
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
     }
-    HIPCHK(hipMemcpyAsync(ring.data(), ctx->R.ring, sizeof(uint32_t) * ring.size(),
+    HIPCHK(hipMemcpyAsync(ring, ctx->R.ring,
+                          sizeof(uint32_t) * kRing * kRingStride + kMiscBytes,
                           hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost,
-                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ProfCollect(ctx);
     ctx->cnt.iterations += K;
     const int rc = CheckMisc(ctx, misc);
     if (rc) return rc;
-    const uint32_t* row = ring.data() + ((iter - 1 + kRing) % kRing) * kRingStride;
+    const uint32_t* row = ring + ((iter - 1 + kRing) % kRing) * kRingStride;
     const uint32_t live = row[kQ] + row[kQ2];  // live queue: Q, then Q2
     if (live == 0) break;
     bound = ctx->fixed_fission && !ctx->R.keig ? ctx->cap : live;
@@ -1484,9 +1490,8 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
                        ctx->stream, ctx->W, R, ctx->src, N);
   rc = RunEvents(ctx, N);
   if (rc) return rc;
-  uint32_t misc[16];
-  HIPCHK(hipMemcpyAsync(misc, R.misc, sizeof(misc), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // RunEvents ends with a read of the counters after its last kernel
+  const uint32_t* misc = ctx->host_rb + kRing * kRingStride;
   const uint32_t M = misc[2];
   // sort the bank by key = history * 64 + child
   rc = Scan(ctx, R.fis_cnt, ctx->fis_off, N);
