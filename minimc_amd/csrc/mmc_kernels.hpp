@@ -84,6 +84,27 @@ __device__ __forceinline__ uint32_t WavePush(uint32_t* counter, bool pred,
   base = __shfl(base, 0);
   return base + prefix;
 }
+// A push of the whole block with one atomic: the waves' counts are summed in
+// LDS (at most 16 waves).  Every thread of the block must call it, once.
+__device__ __forceinline__ uint32_t BlockPush(uint32_t* counter, bool pred) {
+  __shared__ uint32_t wave_base[17];
+  const uint64_t mask = __ballot(pred);
+  const uint32_t lane = __lane_id();
+  const int wave = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
+  if (lane == 0) wave_base[wave] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (int w = 0; w < nw; w++) {
+      const uint32_t c = wave_base[w];
+      wave_base[w] = total;
+      total += c;
+    }
+    wave_base[16] = total ? atomicAdd(counter, total) : 0u;
+  }
+  __syncthreads();
+  return wave_base[16] + wave_base[wave] + __popcll(mask & ((1ull << lane) - 1ull));
+}
 // Two pushes of one wave with one atomic: counter pair c[0] (pred0) and
 // c[1] (pred1), adjacent u32s at an 8-byte aligned address, updated as one
 // u64 (low word c[0]; no count comes near 2^32, so no carry crosses).
@@ -246,8 +267,7 @@ __global__ void KSourceFixed(DevWorld W, RunState R, mmc_source_desc s, uint64_t
   ClearIndirect(W, R, i);
   ok = cell >= 0 && W.cells[cell].material >= 0;  // born in void: leaks at once
   }
-  uint32_t tot;
-  const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
+  const uint32_t iq = BlockPush(Ring(R, -1, kQ), ok);
   if (ok) R.Q[0][iq] = i;
 }
 
@@ -284,8 +304,7 @@ __global__ void KSourceSites(DevWorld Wg, RunState R, const mmc_site* src, uint3
   R.fis_cnt[i] = 0;
   ok = cell >= 0 && W.cells[cell].material >= 0;
   }
-  uint32_t tot;
-  const uint32_t iq = WavePush(Ring(R, -1, kQ), ok, tot);
+  const uint32_t iq = BlockPush(Ring(R, -1, kQ), ok);
   if (ok) R.Q[0][iq] = i;
 }
 
