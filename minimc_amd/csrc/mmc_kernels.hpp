@@ -122,6 +122,42 @@ __device__ __forceinline__ void WavePush2(uint32_t* c, bool pred0, bool pred1, u
   i1 = b1 + __popcll(m1 & below);
 }
 static_assert(kQ % 2 == 0 && kC == kQ + 1 && kRingStride % 2 == 0, "kQ, kC: one u64");
+// WavePush2 for the whole block: one atomic per block and call.  Every
+// thread of the block must call it (in a grid-stride loop, the pass count is
+// the same for the whole block).  Between two calls, a wave writes only its
+// own count, and thread 0 rewrites the shared entries only after every wave
+// has passed the first barrier, i.e. has read the previous call's bases.
+__device__ __forceinline__ void BlockPush2(uint32_t* c, bool pred0, bool pred1, uint32_t& i0,
+                                           uint32_t& i1) {
+  __shared__ uint32_t n0[17], n1[17];  // per-wave counts, then offsets; [16]: block base
+  const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1);
+  const uint32_t lane = __lane_id();
+  const int wave = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
+  if (lane == 0) {
+    n0[wave] = __popcll(m0);
+    n1[wave] = __popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t0 = 0, t1 = 0;
+    for (int w = 0; w < nw; w++) {
+      const uint32_t a = n0[w], b = n1[w];
+      n0[w] = t0;
+      n1[w] = t1;
+      t0 += a;
+      t1 += b;
+    }
+    uint64_t base = 0;
+    if (t0 | t1)
+      base = atomicAdd(reinterpret_cast<unsigned long long*>(c), (uint64_t(t1) << 32) | t0);
+    n0[16] = uint32_t(base);
+    n1[16] = uint32_t(base >> 32);
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  i0 = n0[16] + n0[wave] + __popcll(m0 & below);
+  i1 = n1[16] + n1[wave] + __popcll(m1 & below);
+}
 static_assert(kT % 2 == 0 && kQ2 == kT + 1, "kT, kQ2: one u64");
 // A wave's event count over all its grid-stride passes, added to the device
 // counter once when the wave is done: one atomic per wave and counter, not
@@ -589,7 +625,7 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
     const uint32_t slot = kMove ? q : p;  // the particle's slot after this kernel
     const bool live = o.to_q || o.x_alive;
     uint32_t iq, ic;
-    WavePush2(Ring(R, iter, kQ), live, o.to_c, iq, ic);
+    WavePush2(Ring(R, iter, kQ), live, o.to_c, iq, ic);  // (per block: measured slower here)
     if (o.to_c) R.QC[ic] = slot;
     if (live) R.Q[qin ^ 1][iq] = slot;
     t_adv.add(valid);
@@ -927,7 +963,7 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
       CollideEvent(W, R, p, hash, Ht, R.Q[qin ^ 1], Ring(R, iter, kQ), alive, to_t);
     }
     uint32_t it, iq;
-    WavePush2(Ring(R, iter, kT), to_t, alive, it, iq);
+    BlockPush2(Ring(R, iter, kT), to_t, alive, it, iq);
     if (to_t) R.QT[it] = p;
     if (alive) R.Q2[qin ^ 1][iq] = p;
     t_col.add(valid);
