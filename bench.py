@@ -65,14 +65,40 @@ BYTES = {
 MLP_FLOP_PER_SAMPLE = 2 * (4 * 256 + 256 * 256 + 256 * 256 + 256 * 2)
 
 
-def cpu_baseline(cfg, threads, target_s=12.0):
-    """The oracle (CPU restatement, `threads` workers) on a bounded sample."""
+def host_cpu():
+    """The host's processor as `nproc` and `lscpu` report it."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core", "CPU(s)", "CPU max MHz"):
+                info["lscpu " + k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baseline(cfg, threads, target_s=12.0, single_s=6.0):
+    """The oracle (CPU restatement) on a bounded sample of the same workload:
+    `threads` workers (the reference's scheme, FixedSource.cpp:22-77), then
+    one worker for the single-thread rate."""
+    multi = cpu_rate(cfg, threads, target_s)
+    single = cpu_rate(cfg, 1, single_s)
+    multi["single_thread"] = {k: single[k] for k in ("value", "unit", "sample")}
+    multi["host"] = host_cpu()
+    return multi
+
+
+def cpu_rate(cfg, threads, target_s):
     from oracle.oracle import Oracle
     from minimc_amd import configs
     if cfg == "c2":
         pr = configs.c2_slab()
         o = Oracle(pr)
-        n = 2000
+        n = 200 * threads
         t0 = time.perf_counter()
         o.run_fixed_source(n, threads=threads)
         dt = time.perf_counter() - t0
@@ -88,11 +114,11 @@ def cpu_baseline(cfg, threads, target_s=12.0):
         # POD sampler (the reference's method); 2 untimed inactive cycles, then
         # active cycles until about target_s of work is timed
         pr = configs.c5_multizone()
-        n = 100_000
+        n = 100_000 if threads > 4 else 10_000
         o = Oracle(pr)
         o.set_entropy_mesh(*pr.entropy_mesh)
         o.keig_init(n)
-        for _ in range(2):
+        for _ in range(2 if threads > 4 else 1):
             o.run_cycle(active=False, threads=threads)
         cycles, hist, dt = 0, 0, 0.0
         while dt < target_s and cycles < 200:
@@ -121,10 +147,28 @@ def _profiled_traffic(kernel, workload):
     return None
 
 
+def _tail_roofline(cfg, kt, ev):
+    """The history-based tail kernel against the HBM roofline: the
+    algorithmic bytes of the events it ran (the event kernels' per-event
+    bytes) over its HIP-event time."""
+    if "tail" not in kt or kt["tail"][1] == 0:
+        return None
+    ms, launches = kt["tail"]
+    b = BYTES[cfg]
+    nbytes = (ev["tail_advances"] * b["advance"] + ev["tail_crossings"] * b["cross"] +
+              ev["tail_collisions"] * b["collide"] + ev["tail_tsl"] * b["tsl"])
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "tail", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "events": {k: ev["tail_" + k] for k in ("advances", "crossings", "collisions",
+                                                    "tsl")},
+            "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
+            "avg_launch_ms": ms / max(launches, 1)}
+
+
 def _roofline(cfg, kt, counts):
     """Roofline of the event kernel with the largest total time (the
-    history-based tail kernel is latency-bound by construction and is
-    reported beside, not here)."""
+    history-based tail kernel is reported beside it: _tail_roofline)."""
     dom = max((k for k in kt if k != "tail"), key=lambda k: kt[k][0])
     ms, launches = kt[dom]
     if dom == "mlp":
@@ -184,7 +228,58 @@ def run_c2(args):
         "leakage_probability": p, "collisions_per_s": ev["collisions"] / dt,
         "events": counts, "kernel_ms": {k: v[0] for k, v in kt.items()},
         "roofline": _roofline("c2", kt, counts),
+        "tail_roofline": _tail_roofline("c2", kt, {k: c1[k] - c0[k] for k in c1}),
     }
+
+
+FOM_BATCH = 5  # cycles per batch mean (cycles correlate through the source)
+
+
+def batch_means(ks, batch=FOM_BATCH):
+    nb = len(ks) // batch
+    return np.asarray(ks[:nb * batch]).reshape(nb, batch).mean(1)
+
+
+def fom_of(bm, seconds):
+    """1/(sigma_rel^2 T) with sigma from batch means bm of the cycle k."""
+    sig = bm.std(ddof=1) / np.sqrt(len(bm))
+    return 1.0 / ((sig / bm.mean()) ** 2 * seconds)
+
+
+def fom_ratio_interval(ks_a, t_a, ks_b, t_b, reps=4000, seed=1):
+    """16th, 50th and 84th percentiles of FOM_a / FOM_b under a bootstrap of
+    each run's batch means (block bootstrap over batches of cycles)."""
+    rng = np.random.default_rng(seed)
+    a, b = batch_means(ks_a), batch_means(ks_b)
+    if len(a) < 3 or len(b) < 3:
+        return None
+    r = np.empty(reps)
+    for i in range(reps):
+        r[i] = fom_of(rng.choice(a, len(a)), t_a) / fom_of(rng.choice(b, len(b)), t_b)
+    return [float(v) for v in np.percentile(r, [16, 50, 84])]
+
+
+def entropy_check(h_inactive, h_active):
+    """Source convergence: the Shannon entropy of the last inactive cycles
+    against the active cycles' mean and spread, and the drift of the active
+    cycles' entropy (least-squares slope per cycle, with its standard error)."""
+    ha = np.asarray(h_active)
+    tail = np.asarray(h_inactive[-5:]) if len(h_inactive) else ha[:1]
+    sd = ha.std(ddof=1) if len(ha) > 1 else 0.0
+    x = np.arange(len(ha), dtype=float)
+    slope, se = 0.0, 0.0
+    if len(ha) > 2:
+        A = np.stack([x, np.ones_like(x)], 1)
+        coef, res, *_ = np.linalg.lstsq(A, ha, rcond=None)
+        slope = float(coef[0])
+        resid = ha - A @ coef
+        se = float(np.sqrt(resid @ resid / (len(ha) - 2) / ((x - x.mean()) ** 2).sum()))
+    gap = float(tail.mean() - ha.mean())
+    return {"inactive_last5_mean": float(tail.mean()), "active_mean": float(ha.mean()),
+            "active_sd": float(sd), "gap": gap, "active_slope_per_cycle": slope,
+            "active_slope_se": se,
+            "converged": bool(abs(gap) <= 3 * sd / np.sqrt(len(tail)) + 1e-12 and
+                              abs(slope) <= 3 * se + 1e-12)}
 
 
 def _keig_leg(pr, tracker, warmup, steps):
@@ -192,8 +287,9 @@ def _keig_leg(pr, tracker, warmup, steps):
     t = tracker
     t.set_entropy_mesh(*pr.entropy_mesh)
     t.keig_init(pr.histories)
+    h_inactive = []
     for _ in range(warmup):
-        t.run_cycle(active=False)
+        h_inactive.append(t.run_cycle(active=False)["entropy"])
     c0 = t.counters()
     t.set_profiling(True)
     ks, ents = [], []
@@ -211,10 +307,15 @@ def _keig_leg(pr, tracker, warmup, steps):
     k = float(ks.mean())
     sig = float(ks.std(ddof=1) / np.sqrt(len(ks))) if len(ks) > 1 else float("nan")
     rel = sig / k
+    bm = batch_means(ks)
     return {"seconds": dt, "k": k, "sigma_k": sig, "fom": 1.0 / (rel * rel * dt),
+            "sigma_k_batch_means": float(bm.std(ddof=1) / np.sqrt(len(bm)))
+            if len(bm) > 1 else None,
+            "fom_batch_means": float(fom_of(bm, dt)) if len(bm) > 2 else None,
             "histories_per_s": steps * pr.histories / dt,
             "collisions_per_s": ev["collisions"] / dt,
-            "entropy_last": ents[-1], "events": ev,
+            "entropy_last": ents[-1], "entropy": entropy_check(h_inactive, ents),
+            "k_cycles": [float(v) for v in ks], "events": ev,
             "kernel_ms": {n: v[0] for n, v in kt.items() if v[1] > 0},
             "kernel_launches": {n: v[1] for n, v in kt.items() if v[1] > 0},
             "_kt": kt}
@@ -277,13 +378,19 @@ def run_c5(args):
     ts.close()
     ev = sur["events"]
     kt = sur.pop("_kt")
-    pod.pop("_kt")
-    counts = {"advance": ev["advances"], "collide": ev["collisions"],
-              "cross": ev["crossings"],
-              "tsl": ev["tsl_collisions"] - ev["tsl_fallbacks"],
-              "mlp": ev["tsl_collisions"] - ev["tsl_fallbacks"]}
+    pod_kt = pod.pop("_kt")
+    # the surrogate run's event-kernel rows: the network sampled mlp_rows
+    # thermal collisions; the POD sampler the parked fallbacks (in the
+    # thermal-scattering kind) and every thermal collision of the tail kernel
+    counts = {"advance": ev["advances"] - ev["tail_advances"],
+              "collide": ev["collisions"] - ev["tail_collisions"],
+              "cross": ev["crossings"] - ev["tail_crossings"],
+              "tsl": ev["tsl_fallbacks"], "mlp": ev["mlp_rows"]}
     kt = {k: v for k, v in kt.items() if v[1] > 0}
+    pod_kt = {k: v for k, v in pod_kt.items() if v[1] > 0}
     dk = abs(sur["k"] - pod["k"]) / np.hypot(sur["sigma_k"], pod["sigma_k"])
+    ks_s, ks_p = sur.pop("k_cycles"), pod.pop("k_cycles")
+    ratio_ci = fom_ratio_interval(ks_s, sur["seconds"], ks_p, pod["seconds"])
     return {
         "value": sur["histories_per_s"], "unit": "histories/s",
         "ms_per_step": sur["seconds"] * 1e3 / args.steps, "dtype": "f64",
@@ -294,6 +401,17 @@ def run_c5(args):
                    "histories_per_cycle": n, "inactive_cycles": args.warmup,
                    "active_cycles": args.steps},
         "fom_ratio": sur["fom"] / pod["fom"],
+        "fom_ratio_batch_means": (sur["fom_batch_means"] / pod["fom_batch_means"]
+                                  if sur["fom_batch_means"] and pod["fom_batch_means"]
+                                  else None),
+        "fom_ratio_68pct": ratio_ci,
+        "fom_note": "FOM = 1/(sigma_rel(k)^2 T) over the active cycles; sigma from "
+                    "batch means of %d cycles; the interval is a bootstrap over "
+                    "batches of both runs" % FOM_BATCH,
+        "thermal_sampling": {"network_rows": ev["mlp_rows"],
+                             "pod_fallbacks": ev["tsl_fallbacks"],
+                             "pod_in_tail_kernel": ev["tail_tsl"],
+                             "network_fraction": ev["mlp_rows"] / max(ev["tsl_collisions"], 1)},
         "histories_per_s_ratio": sur["histories_per_s"] / pod["histories_per_s"],
         "k_diff_sigma": float(dk),
         "surrogate": sur, "pod": pod, "tsl_batch": batch,
@@ -302,6 +420,8 @@ def run_c5(args):
         "network": {"trained_here": trained_here, "seconds": t_net,
                     "path": os.path.relpath(net_path, ROOT)},
         "roofline": _roofline("c5", kt, counts),
+        "tail_roofline": _tail_roofline("c5", kt, ev),
+        "pod_tail_roofline": _tail_roofline("c5", pod_kt, pod["events"]),
     }
 
 
@@ -339,7 +459,7 @@ def main():
            "data": "synthetic"}
     out.update(res)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(args.config, threads)
     else:
         out["cpu_baseline"] = None

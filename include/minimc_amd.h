@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MMC_ABI_VERSION 1
+#define MMC_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum mmc_status {
@@ -357,6 +357,9 @@ typedef struct mmc_counters {
   uint64_t iterations;   /* event-loop iterations                       */
   uint64_t tail_particles; /* particles finished by the fused tail kernel */
   double transport_seconds;
+  /* events run inside the tail kernel (also counted above) */
+  uint64_t tail_advances, tail_collisions, tail_tsl, tail_crossings;
+  uint64_t mlp_rows;     /* thermal collisions sampled by the network     */
 } mmc_counters;
 
 /* ---- k-eigenvalue cycle statistics -------------------------------------- */

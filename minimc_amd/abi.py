@@ -172,7 +172,9 @@ class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "histories", "xs_lookups", "advances", "collisions", "tsl_collisions",
         "tsl_fallbacks", "crossings", "virtual_collisions", "fission_sites",
-        "iterations", "tail_particles")] + [("transport_seconds", C.c_double)]
+        "iterations", "tail_particles")] + [("transport_seconds", C.c_double)] + [
+        (n, C.c_uint64) for n in ("tail_advances", "tail_collisions", "tail_tsl",
+                                  "tail_crossings", "mlp_rows")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

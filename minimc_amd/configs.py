@@ -316,3 +316,138 @@ def c5_multizone(histories=1_000_000, fuel_aden=0.005, mod_aden=H_ADEN, refl_ade
     pr.entropy_mesh = ([8, 8, 8], [lo, lo, lo], [hi, hi, hi])
     pr.cell_temperatures = temps
     return pr
+
+
+# ---------------------------------------------------------------------------
+# Problems that exercise single features of the path, with closed forms.
+
+FREE_GAS_EVAL_T = ROOM_TEMPERATURE
+
+
+def free_gas_hydrogen(sigma_s=20.0, temperature=FREE_GAS_EVAL_T):
+    """A pure scatterer of hydrogen mass (awr < 1: the free-gas adjustment
+    applies at every energy, ContinuousReaction.cpp:207-223), constant 20 b
+    evaluated at `temperature`, no total table: its total depends on the
+    cell temperature through the free-gas factor alone."""
+    from . import synth
+    E = synth.log_grid(400, 1e-11, 20.0)
+    return dict(awr=H_AWR, reactions=[
+        dict(kind=abi.MMC_RX_SCATTER, xs=(E, np.full_like(E, sigma_s)),
+             temperature=temperature)])
+
+
+def linear_temperature_slab(width=2.0, T0=300.0, T1=1200.0, E=1e-8, aden=H_ADEN,
+                            tracking=abi.MMC_CELL_DELTA_TRACKING):
+    """Slab 0 <= x < width of the free-gas hydrogen scatterer above, with the
+    linear temperature field T(x) = T0 + (T1 - T0) x / width (bounds T0, T1:
+    ScalarField.cpp:52-58).  A pencil beam enters at x = 0 along +x at
+    energy E (0.01 eV).  The transmitted current is binned in cosine with the
+    beam direction: an uncollided particle has cosine exactly 1 and lands in
+    the upper overflow bin, so that bin counts the uncollided transmission.
+    The free-gas total grows with T, so the cell majorant (at T1) exceeds the
+    total everywhere but at x = width: delta tracking makes virtual
+    collisions."""
+    pr = Problem(abi.MMC_CONTINUOUS)
+    pr.add_field()
+    g = (T1 - T0) / width
+    f = pr.add_field(kind=abi.MMC_FIELD_LINEAR, c=T0, g=(g, 0.0, 0.0), b=T0, lower=T0,
+                     upper=T1)
+    s0 = pr.add_surface("left", abi.MMC_PLANEX, a=0.0)
+    s1 = pr.add_surface("right", abi.MMC_PLANEX, a=width)
+    pr.nuclides.append(free_gas_hydrogen())
+    m = pr.add_material("hydrogen", aden, [(0, 1.0)])
+    pr.add_cell("slab", m, [(s0, False), (s1, True)], f)
+    pr.add_cell("", -1, [(s0, True)])
+    pr.add_cell("", -1, [(s1, False)])
+    pr.tracking = tracking
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_CONSTANT,
+                     direction=(1.0, 0.0, 0.0), E=E)
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="transmitted", surface=s1,
+                              cosine=dict(direction=(1, 0, 0),
+                                          bins=dict(kind=abi.MMC_BINS_LINSPACE, min=-1.0,
+                                                    max=1.0, bins=2))))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="reflected", surface=s0))
+    pr.field_T = (T0, T1, width)
+    return pr
+
+
+def continuous_temperature_slab(tables=None, T0=300.0, T1=600.0, width=4.33,
+                                tracking=abi.MMC_CELL_DELTA_TRACKING):
+    """The reference's continuous-temperature benchmark
+    (benchmarks/continuous_temperature.xml) on the synthetic thermal
+    scattering law: a 4.33 cm slab of bound hydrogen (planes at x = -1e-6 and
+    4.33), linear temperature 300 K at x = 0 rising to 600 K at x = 4.33 cm
+    (gradient 69.28406467 K/cm), cell delta tracking, a pencil beam at
+    0.56 eV along +x from the origin; leakage through the far plane in the
+    benchmark's 200 energy bins (boundaries e^(-25 + 0.06 k) MeV) and through
+    the near plane.  T1 == T0 gives a constant-temperature slab."""
+    t = tables if tables is not None else tsl_tables()
+    pr = Problem(abi.MMC_CONTINUOUS)
+    pr.add_field()
+    g = (T1 - T0) / width
+    f = pr.add_field(kind=abi.MMC_FIELD_LINEAR if T1 != T0 else abi.MMC_FIELD_CONSTANT,
+                     c=T0, g=(g, 0.0, 0.0), b=T0, lower=T0, upper=T1)
+    s0 = pr.add_surface("plane-1", abi.MMC_PLANEX, a=-1e-6)
+    s1 = pr.add_surface("plane-14", abi.MMC_PLANEX, a=width)
+    pr.tsl.append(t)
+    pr.nuclides.append(h_in_water(0))
+    m = pr.add_material("hydrogen in water", H_ADEN, [(0, 1.0)])
+    pr.add_cell("", -1, [(s0, True)])
+    pr.add_cell("segment-1", m, [(s0, False), (s1, True)], f)
+    pr.add_cell("", -1, [(s1, False)])
+    pr.tracking = tracking
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_CONSTANT,
+                     direction=(1.0, 0.0, 0.0), E=0.56e-6)
+    bounds = [float(np.exp(-25.0 + 0.06 * k)) for k in range(201)]
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="leakage", surface=s1,
+                              energy=dict(kind=abi.MMC_BINS_BOUNDARIES, bounds=bounds)))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="reflected", surface=s0))
+    return pr
+
+
+def absorber_slab(width=1.0, sigma=1.0, dir_kind=abi.MMC_DIR_ISOTROPIC_FLUX,
+                  direction=(1.0, 0.0, 0.0)):
+    """One-group pure absorber 0 <= x < width (Sigma = sigma / cm), source at
+    the origin on the near face.  With the isotropic-flux source
+    (IsotropicFlux::Sample, Source.cpp:124-129: p(mu) = 2 mu about +x) the
+    transmitted fraction is 2 E_3(sigma width)."""
+    pr = Problem(abi.MMC_MULTIGROUP, 1)
+    pr.add_field()
+    s0 = pr.add_surface("left", abi.MMC_PLANEX, a=0.0)
+    s1 = pr.add_surface("right", abi.MMC_PLANEX, a=width)
+    pr.nuclides.append(dict(capture=np.array([sigma])))
+    m = pr.add_material("absorber", 1.0, [(0, 1.0)])
+    pr.add_cell("slab", m, [(s0, False), (s1, True)])
+    pr.add_cell("", -1, [(s0, True)])
+    pr.add_cell("", -1, [(s1, False)])
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=dir_kind, direction=direction, E=1.0)
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="transmitted", surface=s1))
+    pr.estimators.append(dict(kind=abi.MMC_EST_CURRENT, name="reflected", surface=s0))
+    return pr
+
+
+# Three two-group nuclides told apart by what a collision does: A captures,
+# B scatters into group 1, C scatters into group 2.
+MIX_SIGMA = (1.0, 2.0, 3.0)
+MIX_AFRAC = (5.0, 3.0, 2.0)  # normalised by the material (Material.cpp:82-92)
+
+
+def nuclide_mixture():
+    """A two-group material of three nuclides whose collisions tell them
+    apart: A only captures (sigma 1), B only scatters 1 -> 1 (sigma 2), C
+    only scatters 1 -> 2 (sigma 3).  Particle::SampleNuclide
+    (Particle.cpp:110-124) picks nuclide i with probability
+    afrac_i sigma_i / sum_j afrac_j sigma_j."""
+    pr = Problem(abi.MMC_MULTIGROUP, 2)
+    pr.add_field()
+    s = pr.add_surface("s", abi.MMC_SPHERE, r=1e6)
+    a, b, c = MIX_SIGMA
+    pr.nuclides.append(dict(capture=np.array([a, a])))
+    # scatter matrices in XML order: row = outgoing, column = incoming
+    pr.nuclides.append(dict(scatter=np.array([b, b, 0.0, 0.0])))
+    pr.nuclides.append(dict(scatter=np.array([0.0, 0.0, c, c])))
+    m = pr.add_material("mix", 1.0, list(zip(range(3), MIX_AFRAC)))
+    pr.add_cell("c", m, [(s, True)])
+    pr.add_cell("", -1, [(s, False)])
+    pr.source = dict(position=(0.0, 0.0, 0.0), dir_kind=abi.MMC_DIR_ISOTROPIC, E=1.0)
+    return pr

@@ -31,7 +31,8 @@ enum : uint32_t {
   kErrSteps = 4u,
   kErrAlphaRange = 8u,
   kErrOverflow = 16u,
-  kErrReaction = 32u  // no reaction selected after kMaxReactionResamples
+  kErrReaction = 32u,  // no reaction selected after kMaxReactionResamples
+  kErrRecords = 64u    // more score records than the record buffer holds
 };
 constexpr int kMaxReactionResamples = 1000;
 

@@ -23,7 +23,10 @@ constexpr int kMiscFallback = 5;
 // event counters (u64)
 enum Ev : int {
   kEvAdvance = 0, kEvCollide, kEvCross, kEvVirtual, kEvTsl, kEvFallback,
-  kEvSites, kEvTail, kEvN
+  kEvSites, kEvTail,
+  // the tail kernel's own events (also in the totals above), and the thermal
+  // collisions the network sampled
+  kEvTailAdv, kEvTailCol, kEvTailTsl, kEvTailCross, kEvMlp, kEvN
 };
 
 struct DevBank {
@@ -56,12 +59,11 @@ struct RunState {
   int keig;
   uint32_t max_steps;
   uint32_t chunk_first;  // first history of the chunk (fixed source)
-  // tallies
-  uint32_t* rec_hist;
-  uint32_t* rec_bin;
+  // tallies: one record per score, key = bin << 32 | history (relative to
+  // chunk_first), reduced in a fixed order when the batch ends
+  uint64_t* rec_key;
   double* rec_val;
   uint32_t rec_cap;
-  double* cycS;        // k-eig cycle sums per global bin
   // fission sites (k-eig)
   mmc_site* sites;
   uint32_t site_cap;
@@ -189,30 +191,29 @@ __device__ __forceinline__ V3 LoadPos(const DevBank& B, uint32_t p) {
 __device__ __forceinline__ V3 LoadDir(const DevBank& B, uint32_t p) {
   return {B.u[p], B.v[p], B.w[p]};
 }
-// A new particle's indirect effects start at zero (Particle::SetPerturbations,
-// Particle.cpp:71-73).  The reference gives them only to source particles
-// (FixedSource.cpp:64-65); secondaries have none there and abort a
-// sensitivity tally, here they start at zero.
+// A new particle's indirect effects start at zero: the reference calls
+// Particle::SetPerturbations (Particle.cpp:71-73) on every particle it takes
+// from the bank, source particles and secondaries alike (FixedSource.cpp:
+// 63-65), and PerturbationSet::GetIndirectEffects gives each perturbation
+// the value 0 (Perturbation.cpp:122-128).
 __device__ __forceinline__ void ClearIndirect(const DevWorld& W, const RunState& R,
                                               uint32_t slot) {
   for (int j = 0; j < W.n_pert; j++) R.B.ie[size_t(j) * R.capacity + slot] = 0;
 }
 
-// Score one value into a global tally bin: per-history records for fixed
-// source, cycle sums for k-eigenvalue.
+// Score one value into a global tally bin: a record (bin, history, value).
+// The batch's records are reduced in a fixed order when it ends (per-history
+// sums, then per-bin sums: TallyReduce), so the tallies do not depend on the
+// order in which the GPU produced the scores.  misc[1] counts the records
+// asked for, also past the capacity, so that the caller can size a rerun.
 __device__ __forceinline__ void ScoreBin(const RunState& R, uint32_t hist, int bin,
                                          double val) {
-  if (R.keig) {
-    unsafeAtomicAdd(R.cycS + bin, val);
-    return;
-  }
   const uint32_t k = atomicAdd(R.misc + 1, 1u);
   if (k >= R.rec_cap) {
-    atomicOr(R.misc, kErrOverflow);
+    atomicOr(R.misc, kErrRecords);
     return;
   }
-  R.rec_hist[k] = hist - R.chunk_first;
-  R.rec_bin[k] = uint32_t(bin);
+  R.rec_key[k] = (uint64_t(uint32_t(bin)) << 32) | uint64_t(hist - R.chunk_first);
   R.rec_val[k] = val;
 }
 
@@ -648,6 +649,13 @@ __device__ __forceinline__ void BankChild(const DevWorld& W, const RunState& R, 
                                           uint32_t site = kClaimSite) {
   const uint32_t seed = LcgNext(g);  // Particle::BankSecondaries (Particle.cpp:96-100)
   const uint32_t hist = R.B.hist[p];
+  // The site key history * 64 + child holds 64 children per history.  The
+  // World's nu-bar tables are checked to lie below 63 when it is built, so
+  // a fission yields at most 63; this guards the key all the same.
+  if (child >= 64) {
+    atomicOr(R.misc, kErrOverflow);
+    return;
+  }
   if (R.keig) {
     const uint32_t k = site != kClaimSite ? site : atomicAdd(R.misc + 2, 1u);
     if (k >= R.site_cap) {
@@ -1146,6 +1154,10 @@ __global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, 
     atomicAdd(R.ev + kEvCollide, c_coll);
     atomicAdd(R.ev + kEvTsl, c_tsl);
     atomicAdd(R.ev + kEvTail, c_tail);
+    atomicAdd(R.ev + kEvTailAdv, c_adv);
+    atomicAdd(R.ev + kEvTailCol, c_coll);
+    atomicAdd(R.ev + kEvTailTsl, c_tsl);
+    atomicAdd(R.ev + kEvTailCross, c_cross);
   }
 }
 

@@ -29,6 +29,16 @@ MlpState& MmcMlp(mmc_ctx* ctx);
 const DevWorld& MmcWorld(mmc_ctx* ctx);
 int MmcFail(mmc_ctx* ctx, int code, const char* msg);
 int MmcDevice(mmc_ctx* ctx);
+int MmcLaunchCheck(mmc_ctx* ctx, const char* where);
+
+// Waits for the context's stream after a batch of launches; a launch that
+// failed at configuration time or a fault is an error.
+static int SyncChecked(mmc_ctx* ctx, const char* where) {
+  if (const int lc = MmcLaunchCheck(ctx, where)) return lc;
+  if (hipStreamSynchronize(MmcStream(ctx)) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, where);
+  return MMC_OK;
+}
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
@@ -250,7 +260,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     X[lr_io] = Normalize(m, v);
     if (kTrack) pidx[lr_io] = p;
   }
-  WaveTally t_tsl, t_fall;  // tracker: the io waves' event counts
+  WaveTally t_tsl, t_fall, t_net;  // tracker: the io waves' event counts
   for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
     const uint32_t row = it * kFRows + lr_io;
     const bool valid = io_wave && row < n;
@@ -399,10 +409,12 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     if (fall) io.fb[ifb] = p;
     t_tsl.add(valid);
     t_fall.add(fall);
+    t_net.add(valid && !fall);
   }
   if (kTrack && io_wave) {
     t_tsl.flush(io.R.ev + kEvTsl);
     t_fall.flush(io.R.ev + kEvFallback);
+    t_net.flush(io.R.ev + kEvMlp);
   }
 }
 
@@ -995,7 +1007,7 @@ extern "C" int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* d, const f
   m.ready = FusedShape(m);
   const int rc = Repack(ctx);
   if (rc) return rc;
-  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+  return SyncChecked(ctx, "mmc_mlp_set");
 }
 
 extern "C" int mmc_mlp_get(mmc_ctx* ctx, float* params, uint64_t n_params) {
@@ -1017,7 +1029,7 @@ extern "C" int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y)
   if (n > 0xffffffffull) return MmcFail(ctx, MMC_ERR_ARG, "batch too large");
   const int rc = InferChunked(ctx, m, x, y, n);
   if (rc) return rc;
-  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+  return SyncChecked(ctx, "mmc_mlp_infer");
 }
 
 extern "C" int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y) {
@@ -1026,7 +1038,7 @@ extern "C" int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x
   if (n > 0xffffffffull) return MMC_ERR_ARG;
   hipLaunchKernelGGL(KPodMap, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, MmcStream(ctx),
                      MmcWorld(ctx), tsl, uint32_t(n), x, y);
-  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+  return SyncChecked(ctx, "mmc_tsl_pod_map");
 }
 
 // Forward + backward of one minibatch (normalised x, y) into m.grads.
@@ -1136,7 +1148,7 @@ extern "C" int mmc_mlp_train(mmc_ctx* ctx, uint64_t n, const float* x, const flo
   const int rc = Repack(ctx);
   if (rc) return rc;
   if (last_loss) hipMemcpyAsync(last_loss, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
-  return hipStreamSynchronize(s) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+  return SyncChecked(ctx, "mmc_mlp_train");
 }
 
 // Device batch: normalise raw inputs/targets in place (training data prep).
@@ -1146,7 +1158,7 @@ extern "C" int mmc_mlp_normalize(mmc_ctx* ctx, uint64_t n, float* x, float* y) {
   if (!m.params) return MMC_ERR_ARG;
   hipLaunchKernelGGL(KNormalize, dim3(1024), dim3(256), 0, MmcStream(ctx), x, y, size_t(n),
                      DevView(m));
-  return hipStreamSynchronize(MmcStream(ctx)) == hipSuccess ? MMC_OK : MMC_ERR_HIP;
+  return SyncChecked(ctx, "mmc_mlp_normalize");
 }
 
 __global__ void KSurrogateBatch(DevWorld W, MlpState m, int tsl, uint32_t n, const double* E,
@@ -1213,7 +1225,7 @@ extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const
                      uint32_t(n), E, T, rng, yout, alpha, beta, err);
   uint32_t herr = 0;
   hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-  const bool ok = hipStreamSynchronize(s) == hipSuccess;
+  const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
   (void)hipFree(xin);
   (void)hipFree(yout);
   (void)hipFree(err);

@@ -15,6 +15,7 @@
 #define MMC_TRACKER_KERNELS
 #include "mmc_kernels.hpp"
 #include "mmc_mlp.hpp"
+#include "mmc_tally.hpp"
 
 using namespace mmc;
 
@@ -35,6 +36,9 @@ static constexpr size_t kTotLdsMax = 32 * 1024;
 static_assert(kMaxPert == MMC_MAX_PERTURBATIONS, "perturbation limit");
 static constexpr size_t kDescLdsMax = 24 * 1024;  // the World's descriptor block
 static constexpr size_t kLdsPerBlock = 160 * 1024;
+// Largest nu-bar: a fission then yields at most 63 children, and the fission
+// site key (history * 64 + child) keeps the children of one history apart.
+static constexpr double kMaxNubar = 63.0;
 
 struct mmc_ctx {
   std::string err;
@@ -65,9 +69,8 @@ struct mmc_ctx {
   double *S = nullptr, *S2 = nullptr;
   std::vector<int> est_off, est_size;
   std::vector<int> cell_material;  // host copy for argument checks
-  uint32_t *h_cnt = nullptr, *h_off = nullptr, *h_fill = nullptr;
-  uint32_t* rs_bin = nullptr;
-  double* rs_val = nullptr;
+  double *bin_sum = nullptr, *bin_sq = nullptr;  // a batch's per-bin sums (TallyReduce)
+  TallyWork tally{};
   // k-eigenvalue
   mmc_site* src = nullptr;
   mmc_site* sites_sorted = nullptr;
@@ -280,6 +283,23 @@ static int Fail(mmc_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
 }
+// A launch that fails at configuration time (more LDS than a CU holds, a bad
+// grid) runs nothing and leaves its queue rows at zero, which would read as
+// a finished batch.  Every batch of launches is followed by this check
+// before its results are read.
+static int LaunchCheck(mmc_ctx* ctx, const char* where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return Fail(ctx, MMC_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+  return MMC_OK;
+}
+int MmcLaunchCheck(mmc_ctx* ctx, const char* where) { return LaunchCheck(ctx, where); }
+// Static LDS of a kernel (its __shared__ arrays), which adds to the dynamic
+// LDS a launch asks for.
+static size_t StaticLds(const void* fn) {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, fn) == hipSuccess ? a.sharedSizeBytes : 0;
+}
 
 extern "C" int mmc_abi_version(void) { return MMC_ABI_VERSION; }
 
@@ -412,6 +432,10 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
         if (d.capture) cap[n * G + g] = d.capture[g];
         if (d.fission_xs) fis[n * G + g] = d.fission_xs[g];
         if (d.nubar && d.fission_xs) nub[n * G + g] = d.nubar[g];
+        // a fission yields int(nu-bar + U) children, at most 63 (the site
+        // key holds 64 per history)
+        if (!(nub[n * G + g] >= 0 && nub[n * G + g] < kMaxNubar))
+          return Fail(ctx, MMC_ERR_ARG, "nubar must lie in [0, 63)");
       }
       // columns of the XML matrices: incoming group gp, outgoing g
       // (Multigroup.cpp:124-147); normalised per column (:168-186)
@@ -459,6 +483,9 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
         dr.nubar = r.kind == MMC_RX_FISSION ? F.AddTable(r.nubar) : -1;
         if (r.kind == MMC_RX_FISSION) {
           if (dr.nubar < 0) return Fail(ctx, MMC_ERR_ARG, "fission without nubar");
+          for (int i = 0; i < r.nubar.n; i++)
+            if (!(r.nubar.y[i] >= 0 && r.nubar.y[i] < kMaxNubar))
+              return Fail(ctx, MMC_ERR_ARG, "nubar must lie in [0, 63)");
           dn.fissile = 1;
         }
         dr.chi_kind = r.chi_kind;
@@ -678,30 +705,64 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   // The staging kernels put the descriptor block last in their LDS (KAdvance,
   // KCollide: hash, totals, descriptors; KTail: all of them).  Their LDS variants need it to fit.  KTslPod stages only the
   // sampler's tables (hot_lds above).
+  // Each LDS variant must fit beside the kernel's static LDS.
   const size_t desc_lds = size_t(W.desc_bytes) <= kDescLdsMax ? size_t(W.desc_bytes) : 0;
-  if (!desc_lds) {
-    ctx->ev_tot_lds = false;
-    ctx->ev_lds = hash_lds;
-    ctx->tail_lds = 0;
-  } else {
-    ctx->ev_tot_lds = tot_lds <= kTotLdsMax;
-    ctx->ev_lds = ctx->ev_tot_lds ? hash_lds + tot_lds + desc_lds : hash_lds;
+  const size_t ev_static = std::max(
+      {StaticLds(reinterpret_cast<const void*>(KAdvance<true, false>)),
+       StaticLds(reinterpret_cast<const void*>(KAdvance<true, true>)),
+       StaticLds(reinterpret_cast<const void*>(KCollide<true>))});
+  const size_t ev_static0 = std::max(
+      {StaticLds(reinterpret_cast<const void*>(KAdvance<false, false>)),
+       StaticLds(reinterpret_cast<const void*>(KAdvance<false, true>)),
+       StaticLds(reinterpret_cast<const void*>(KCollide<false>))});
+  const size_t tail_static = StaticLds(reinterpret_cast<const void*>(KTail<true>));
+  if (ctx->hot_lds &&
+      ctx->hot_lds + StaticLds(reinterpret_cast<const void*>(KTslPod<true>)) > kLdsPerBlock)
+    ctx->hot_lds = 0;
+  if (hash_lds + ev_static0 > kLdsPerBlock)
+    return Fail(ctx, MMC_ERR_UNSUPPORTED, "energy hash does not fit in LDS");
+  ctx->ev_tot_lds = desc_lds && tot_lds <= kTotLdsMax &&
+                    hash_lds + tot_lds + desc_lds + ev_static <= kLdsPerBlock;
+  ctx->ev_lds = ctx->ev_tot_lds ? hash_lds + tot_lds + desc_lds : hash_lds;
+  {
     const size_t hot = size_t(W.hot_n) * sizeof(double);
     const size_t need = hash_lds + tot_lds + hot + desc_lds;
-    ctx->tail_lds = hot <= kHotLdsMax && need <= kLdsPerBlock ? need : 0;
+    ctx->tail_lds =
+        desc_lds && hot <= kHotLdsMax && need + tail_static <= kLdsPerBlock ? need : 0;
   }
   ctx->S = DevAlloc<double>(ctx, nb);
   ctx->S2 = DevAlloc<double>(ctx, nb);
-  ctx->R.cycS = DevAlloc<double>(ctx, nb);
+  ctx->bin_sum = DevAlloc<double>(ctx, nb);
+  ctx->bin_sq = DevAlloc<double>(ctx, nb);
   HIPCHK(hipMemset(ctx->S, 0, sizeof(double) * std::max(nb, 1)));
   HIPCHK(hipMemset(ctx->S2, 0, sizeof(double) * std::max(nb, 1)));
-  HIPCHK(hipMemset(ctx->R.cycS, 0, sizeof(double) * std::max(nb, 1)));
   // fixed-source problems with fission keep half the slots for children
   if (W.energy_mode == MMC_MULTIGROUP) {
     for (double v : fis) ctx->fixed_fission |= v > 0;
   } else {
     for (const auto& n : ce) ctx->fixed_fission |= n.fissile != 0;
   }
+  return MMC_OK;
+}
+
+// Score record buffers (RunState::rec_key, rec_val) and the reduction's work
+// space for n records.  A k-eigenvalue cycle that scores more grows them and
+// runs again.
+static int EnsureRecords(mmc_ctx* ctx, uint32_t n) {
+  RunState& R = ctx->R;
+  if (R.rec_key && R.rec_cap >= n) return MMC_OK;
+  if (R.rec_key) (void)hipFree(R.rec_key);
+  if (R.rec_val) (void)hipFree(R.rec_val);
+  R.rec_key = nullptr;
+  R.rec_val = nullptr;
+  R.rec_cap = 0;
+  if (hipMalloc(reinterpret_cast<void**>(&R.rec_key), sizeof(uint64_t) * size_t(n)) !=
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&R.rec_val), sizeof(double) * size_t(n)) !=
+          hipSuccess ||
+      TallyWorkEnsure(ctx->tally, n))
+    return Fail(ctx, MMC_ERR_HIP, "out of device memory (score records)");
+  R.rec_cap = n;
   return MMC_OK;
 }
 
@@ -764,15 +825,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
     ctx->host_rb = nullptr;
   R.ev = DevAlloc<unsigned long long>(ctx, kEvN);
   R.capacity = cap;
-  R.rec_cap = cap * 4 + 1024;
-  R.rec_hist = DevAlloc<uint32_t>(ctx, R.rec_cap);
-  R.rec_bin = DevAlloc<uint32_t>(ctx, R.rec_cap);
-  R.rec_val = DevAlloc<double>(ctx, R.rec_cap);
-  ctx->rs_bin = DevAlloc<uint32_t>(ctx, R.rec_cap);
-  ctx->rs_val = DevAlloc<double>(ctx, R.rec_cap);
-  ctx->h_cnt = DevAlloc<uint32_t>(ctx, cap);
-  ctx->h_off = DevAlloc<uint32_t>(ctx, cap);
-  ctx->h_fill = DevAlloc<uint32_t>(ctx, cap);
+  if (const int rr = EnsureRecords(ctx, cap * 4 + 1024)) return rr;
   R.fis_cnt = DevAlloc<uint32_t>(ctx, cap);
   ctx->fis_off = DevAlloc<uint32_t>(ctx, cap);
   ctx->scan_cap = (cap + 1023) / 1024 + 1;
@@ -867,6 +920,9 @@ extern "C" void mmc_destroy(mmc_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void* p : ctx->allocs) hipFree(p);
+  if (ctx->R.rec_key) hipFree(ctx->R.rec_key);
+  if (ctx->R.rec_val) hipFree(ctx->R.rec_val);
+  TallyWorkFree(ctx->tally);
   if (ctx->host_rb) hipHostFree(ctx->host_rb);
   MlpFree(ctx->mlp);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -944,38 +1000,15 @@ static void FillF64(mmc_ctx* ctx, double* p, double v, uint32_t n) {
   hipLaunchKernelGGL(KFillF64, dim3(Blocks(n, 2048)), dim3(kBlock), 0, ctx->stream, p, v, n);
 }
 
-// ---- per-history tally commit (ScorableProxy::CommitHistory, Scorable.cpp:101-106)
-__global__ void KRecCount(const uint32_t* rh, uint32_t nrec, uint32_t* cnt) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrec) atomicAdd(cnt + rh[r], 1u);
-}
-__global__ void KRecPlace(const uint32_t* rh, const uint32_t* rb, const double* rv,
-                          uint32_t nrec, const uint32_t* off, uint32_t* fill,
-                          uint32_t* sb, double* sv) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrec) return;
-  const uint32_t h = rh[r];
-  const uint32_t pos = off[h] + atomicAdd(fill + h, 1u);
-  sb[pos] = rb[r];
-  sv[pos] = rv[r];
-}
-__global__ void KRecCommit(const uint32_t* cnt, const uint32_t* off, const uint32_t* sb,
-                           const double* sv, uint32_t nh, double* S, double* S2) {
-  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nh) return;
-  const uint32_t c = cnt[h];
-  const uint32_t o = off[h];
-  for (uint32_t i = 0; i < c; i++) {
-    const uint32_t b = sb[o + i];
-    bool seen = false;
-    for (uint32_t j = 0; j < i && !seen; j++) seen = sb[o + j] == b;
-    if (seen) continue;
-    double s = 0;
-    for (uint32_t j = i; j < c; j++)
-      if (sb[o + j] == b) s += sv[o + j];
-    unsafeAtomicAdd(S + b, s);
-    unsafeAtomicAdd(S2 + b, s * s);
-  }
+// ---- tally commit (ScorableProxy::CommitHistory, Scorable.cpp:101-106;
+// EstimatorSet::operator+=, Estimator.cpp:215-231): a batch's per-bin sums of
+// s_{h,b} and s_{h,b}^2 (TallyReduce, in a fixed order) join S and S2.
+__global__ void KFoldFixed(double* S, double* S2, const double* sum, const double* sq,
+                           uint32_t nb) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  S[b] += sum[b];
+  S2[b] += sq[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -990,7 +1023,8 @@ static int CheckMisc(mmc_ctx* ctx, const uint32_t* misc) {
   if (e & kErrResample)
     return Fail(ctx, MMC_ERR_RESAMPLE,
                 "Number of thermal scattering resamples exceeded limit: 10");
-  if (e & kErrOverflow) return Fail(ctx, MMC_ERR_OVERFLOW, "bank, queue or record overflow");
+  if (e & kErrOverflow) return Fail(ctx, MMC_ERR_OVERFLOW, "bank, queue or fission-site overflow");
+  if (e & kErrRecords) return Fail(ctx, MMC_ERR_OVERFLOW, "score record overflow");
   if (e & kErrSteps) return Fail(ctx, MMC_ERR_STEPS, "history step limit exceeded");
   if (e & kErrReaction)
     return Fail(ctx, MMC_ERR_ARG, "no reaction selected after 1000 resamples");
@@ -1170,6 +1204,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
         hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                            iter, qin);
       ProfEnd(ctx);
+      if (const int lc = LaunchCheck(ctx, "tail kernel")) return lc;
       HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, kMiscBytes, hipMemcpyDeviceToHost,
                             ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1201,6 +1236,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       ProfEnd(ctx);
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
     }
+    if (const int lc = LaunchCheck(ctx, "event kernels")) return lc;
     HIPCHK(hipMemcpyAsync(ring, ctx->R.ring,
                           sizeof(uint32_t) * kRing * kRingStride + kMiscBytes,
                           hipMemcpyDeviceToHost, ctx->stream));
@@ -1246,6 +1282,11 @@ static int ReadEvents(mmc_ctx* ctx) {
   ctx->cnt.tsl_collisions = ctx->ev_host[kEvTsl];
   ctx->cnt.tsl_fallbacks = ctx->ev_host[kEvFallback];
   ctx->cnt.tail_particles = ctx->ev_host[kEvTail];
+  ctx->cnt.tail_advances = ctx->ev_host[kEvTailAdv];
+  ctx->cnt.tail_collisions = ctx->ev_host[kEvTailCol];
+  ctx->cnt.tail_tsl = ctx->ev_host[kEvTailTsl];
+  ctx->cnt.tail_crossings = ctx->ev_host[kEvTailCross];
+  ctx->cnt.mlp_rows = ctx->ev_host[kEvMlp];
   return MMC_OK;
 }
 
@@ -1289,19 +1330,15 @@ extern "C" int mmc_run_fixed_source(mmc_ctx* ctx, const mmc_source_desc* source,
     HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(misc), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const uint32_t nrec = misc[1];
-    if (nrec > 0) {
-      FillU32(ctx, ctx->h_cnt, 0, m);
-      FillU32(ctx, ctx->h_fill, 0, m);
-      hipLaunchKernelGGL(KRecCount, dim3(Blocks(nrec, 1u << 30)), dim3(kBlock), 0, ctx->stream,
-                         ctx->R.rec_hist, nrec, ctx->h_cnt);
-      rc = Scan(ctx, ctx->h_cnt, ctx->h_off, m);
-      if (rc) return rc;
-      hipLaunchKernelGGL(KRecPlace, dim3(Blocks(nrec, 1u << 30)), dim3(kBlock), 0, ctx->stream,
-                         ctx->R.rec_hist, ctx->R.rec_bin, ctx->R.rec_val, nrec, ctx->h_off,
-                         ctx->h_fill, ctx->rs_bin, ctx->rs_val);
-      hipLaunchKernelGGL(KRecCommit, dim3(Blocks(m, 1u << 30)), dim3(kBlock), 0, ctx->stream,
-                         ctx->h_cnt, ctx->h_off, ctx->rs_bin, ctx->rs_val, m, ctx->S, ctx->S2);
+    if (nrec > 0 && ctx->NB) {
+      if (TallyReduce(ctx->tally, ctx->R.rec_key, reinterpret_cast<uint64_t*>(ctx->R.rec_val),
+                      nrec, ctx->NB, ctx->bin_sum, ctx->bin_sq, ctx->stream))
+        return Fail(ctx, MMC_ERR_HIP, "tally reduction failed");
+      hipLaunchKernelGGL(KFoldFixed, dim3(Blocks(ctx->NB, 1u << 30)), dim3(kBlock), 0,
+                         ctx->stream, ctx->S, ctx->S2, ctx->bin_sum, ctx->bin_sq,
+                         uint32_t(ctx->NB));
     }
+    if (const int lc = LaunchCheck(ctx, "tally commit")) return lc;
     done += m;
     ctx->cnt.histories += m;
   }
@@ -1374,6 +1411,7 @@ extern "C" int mmc_keig_init(mmc_ctx* ctx, const mmc_source_desc* source, uint64
   ctx->N = n;
   hipLaunchKernelGGL(KInitSites, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
                      *source, ctx->src, uint32_t(n));
+  if (const int lc = LaunchCheck(ctx, "initial sites")) return lc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return MMC_OK;
 }
@@ -1478,21 +1516,43 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   const uint32_t N = uint32_t(ctx->N);
   RunState& R = ctx->R;
   R.keig = 1;
-  int rc = ResetRun(ctx);
-  if (rc) return rc;
-  if (ctx->NB) FillF64(ctx, R.cycS, 0.0, ctx->NB);
-  const size_t src_lds = size_t(ctx->W.desc_bytes) <= kDescLdsMax ? size_t(ctx->W.desc_bytes) : 0;
-  if (src_lds)
-    hipLaunchKernelGGL(KSourceSites<true>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), src_lds,
-                       ctx->stream, ctx->W, R, ctx->src, N);
-  else
-    hipLaunchKernelGGL(KSourceSites<false>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0,
-                       ctx->stream, ctx->W, R, ctx->src, N);
-  rc = RunEvents(ctx, N);
-  if (rc) return rc;
+  R.chunk_first = 0;  // score records carry the source site index
+  int rc;
   // RunEvents ends with a read of the counters after its last kernel
   const uint32_t* misc = ctx->host_rb + kRing * kRingStride;
+  const size_t src_lds = size_t(ctx->W.desc_bytes) <= kDescLdsMax ? size_t(ctx->W.desc_bytes) : 0;
+  const mmc_counters cnt_snap = ctx->cnt;
+  unsigned long long ev_snap[kEvN];
+  for (;;) {
+    rc = ResetRun(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ev_snap, R.ev, sizeof(ev_snap), hipMemcpyDeviceToHost, ctx->stream));
+    if (src_lds)
+      hipLaunchKernelGGL(KSourceSites<true>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), src_lds,
+                         ctx->stream, ctx->W, R, ctx->src, N);
+    else
+      hipLaunchKernelGGL(KSourceSites<false>, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0,
+                         ctx->stream, ctx->W, R, ctx->src, N);
+    rc = RunEvents(ctx, N);
+    if (rc == MMC_ERR_OVERFLOW && (misc[0] & ~uint32_t(kErrRecords)) == 0) {
+      // Only the score records overflowed.  The cycle is a function of its
+      // source alone, so it runs again, unchanged, with room for the records
+      // (misc[1] counts the records asked for until the overflow was seen).
+      HIPCHK(hipMemcpyAsync(R.ev, ev_snap, sizeof(ev_snap), hipMemcpyHostToDevice,
+                            ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->cnt = cnt_snap;
+      ctx->err.clear();
+      const uint64_t want = std::max<uint64_t>(2ull * R.rec_cap, uint64_t(misc[1]) * 5 / 4);
+      rc = EnsureRecords(ctx, uint32_t(std::min<uint64_t>(want, 0xfffffff0ull)));
+      if (rc) return rc;
+      continue;
+    }
+    if (rc) return rc;
+    break;
+  }
   const uint32_t M = misc[2];
+  const uint32_t nrec = misc[1];
   // sort the bank by key = history * 64 + child
   rc = Scan(ctx, R.fis_cnt, ctx->fis_off, N);
   if (rc) return rc;
@@ -1513,9 +1573,16 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   const uint32_t nbk = Blocks(N, 1u << 30);
   hipLaunchKernelGGL(KKSum, dim3(nbk), dim3(kBlock), 0, ctx->stream, R.B.k_col, R.B.k_trk,
                      R.B.k_abs, N, ctx->kpart);
-  if (active && ctx->NB)
+  // the cycle's tallies: per-bin sums of the scores in a fixed order, then
+  // S += sum / N and S2 += (sum / N)^2 once per active cycle
+  if (active && ctx->NB) {
+    if (TallyReduce(ctx->tally, R.rec_key, reinterpret_cast<uint64_t*>(R.rec_val), nrec,
+                    ctx->NB, ctx->bin_sum, nullptr, ctx->stream))
+      return Fail(ctx, MMC_ERR_HIP, "tally reduction failed");
     hipLaunchKernelGGL(KFoldCycle, dim3(Blocks(ctx->NB, 1u << 30)), dim3(kBlock), 0,
-                       ctx->stream, ctx->S, ctx->S2, R.cycS, uint32_t(ctx->NB), 1.0 / N);
+                       ctx->stream, ctx->S, ctx->S2, ctx->bin_sum, uint32_t(ctx->NB), 1.0 / N);
+  }
+  if (const int lc = LaunchCheck(ctx, "cycle bank and sums")) return lc;
   std::vector<uint32_t> mc(nmesh);
   std::vector<double> kp(size_t(nbk) * 3);
   HIPCHK(hipMemcpyAsync(mc.data(), ctx->mesh_cnt, sizeof(uint32_t) * nmesh,
@@ -1550,6 +1617,7 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   if (M == 0) return Fail(ctx, MMC_ERR_ARG, "no fission sites banked");
   hipLaunchKernelGGL(KResample, dim3(Blocks(N, 1u << 30)), dim3(kBlock), 0, ctx->stream,
                      ctx->sites_sorted, uint64_t(M), uint64_t(N), ctx->src);
+  if (const int lc = LaunchCheck(ctx, "resample")) return lc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   rc = ReadEvents(ctx);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1741,7 +1809,8 @@ extern "C" int mmc_eval_event(mmc_ctx* ctx, int kind, mmc_particle_soa* p, uint6
   hipMemset(err, 0, sizeof(uint32_t));
   hipLaunchKernelGGL(KEval, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream, ctx->W,
                      kind, s, uint32_t(n), err);
-  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   for (int k = 0; k < 9; k++)
     if (hd[k]) hipMemcpy(hd[k], *sd[k], sizeof(double) * n, hipMemcpyDeviceToHost);
   void* ho[5] = {p->rng, p->cell, p->surface, p->event, p->iout};
@@ -1798,7 +1867,8 @@ extern "C" int mmc_sample_source(mmc_ctx* ctx, const mmc_source_desc* src, uint6
   s.cell = reinterpret_cast<int32_t*>(ibuf + n);
   hipLaunchKernelGGL(KEvalSource, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0, ctx->stream,
                      ctx->W, *src, seed0, s, uint32_t(n));
-  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   double* hd[7] = {p->x, p->y, p->z, p->u, p->v, p->w, p->E};
   for (int k = 0; k < 7; k++)
     if (hd[k]) hipMemcpy(hd[k], *sd[k], sizeof(double) * n, hipMemcpyDeviceToHost);
@@ -1853,6 +1923,7 @@ extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const doubl
     hipLaunchKernelGGL(KTslBatch<false>, dim3(Blocks(n, 1u << 30)), dim3(kBlock), 0,
                        ctx->stream, ctx->W, tsl, uint32_t(n), E, T, rng, alpha, beta,
                        ctx->R.misc);
+  if (const int lc = LaunchCheck(ctx, "POD sampler")) return lc;
   uint32_t misc[16] = {0};
   HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, sizeof(uint32_t), hipMemcpyDeviceToHost,
                         ctx->stream));

@@ -250,9 +250,19 @@ def _floats(node, path, expected):
     return np.array(vals)
 
 
+def _resolve(path, base):
+    return path if os.path.isabs(path) else os.path.join(base, path)
+
+
 def _load_table(path, base):
-    """A ContinuousMap table: a .npy (n x 2: key, value) or .npz (x, y)."""
-    p = path if os.path.isabs(path) else os.path.join(base, path)
+    """A ContinuousMap table: a pandas-HDF5 file of one level (the
+    reference's format, HDF5DataSet<1>::ToContinuousMap), or a .npy (n x 2:
+    key, value) or .npz (x, y)."""
+    p = _resolve(path, base)
+    if p.endswith((".hdf5", ".h5")):
+        from . import pandas_h5
+        x, y = pandas_h5.read_table(p)
+        return _f64(x), _f64(y)
     if not os.path.exists(p):
         raise RuntimeError("File not found: " + path)
     if p.endswith(".npz"):
@@ -533,9 +543,58 @@ def _nuclide_from_xml(node, multigroup, G, base, pr):
                 rx["chi_kind"] = abi.MMC_CHI_WATT
                 rx["watt_a"], rx["watt_b"] = float(watt.get("a")), float(watt.get("b"))
         if r.tag == "scatter" and r.find("tsl") is not None:
-            from .tsl_data import load_tsl_npz
-            t = r.find("tsl")
-            pr.tsl.append(load_tsl_npz(os.path.join(base, t.get("file"))))
+            pr.tsl.append(_tsl_from_xml(r.find("tsl"), base))
             rx["tsl"] = len(pr.tsl) - 1
         n["reactions"].append(rx)
     return n
+
+
+def _tsl_from_xml(t, base):
+    """The thermal-scattering law of a <tsl> element (minimc_basictypes.xsd:
+    96-133; ThermalScattering::ThermalScattering, ThermalScattering.cpp:24-99):
+    pandas-HDF5 files for the majorant (E), total_E (E, order), total_T
+    (T, order) and total_S (order), and per partition CDF (F, order), S
+    (order) and E_T (E, T, order) for beta or beta_T (beta, T, order) for
+    alpha.  A `file` attribute instead names one .npz holding all of them
+    (tsl_data.save_tsl_npz)."""
+    if t.get("file") is not None:
+        from .tsl_data import load_tsl_npz
+        return load_tsl_npz(_resolve(t.get("file"), base))
+    from . import pandas_h5
+
+    def rd(node, attr, D):
+        v = node.get(attr)
+        if v is None:
+            raise RuntimeError('tsl: attribute "%s" missing' % attr)
+        return pandas_h5.read(_resolve(v, base), D)
+
+    (mE,), mv = rd(t, "majorant", 1)
+    (E_axis, _), tot_E = rd(t, "total_E", 2)
+    (T_axis, _), tot_T = rd(t, "total_T", 2)
+    _, tot_S = rd(t, "total_S", 1)
+    rank = len(tot_S)
+    if tot_E.shape[1] != rank or tot_T.shape[1] != rank:
+        raise RuntimeError("tsl: total_E, total_T and total_S disagree on the rank")
+
+    def parts(tag, x_attr):
+        node = t.find(tag)
+        if node is None or len(node) == 0:
+            raise RuntimeError('tsl: "%s" needs at least one partition' % tag)
+        out = []
+        for p in node:
+            (F, _), cdf = rd(p, "CDF", 2)
+            _, S = rd(p, "S", 1)
+            (x, T, _), xt = rd(p, x_attr, 3)
+            if cdf.shape[1] != len(S) or xt.shape[2] != len(S):
+                raise RuntimeError("tsl %s: CDF, S and %s disagree on the rank"
+                                   % (tag, x_attr))
+            out.append(dict(F=F, cdf_modes=np.ascontiguousarray(cdf), S=S, x=x, T=T,
+                            x_T_modes=np.ascontiguousarray(xt)))
+        return out
+
+    return dict(majorant=(mE, mv), total_E_axis=E_axis, total_E=np.ascontiguousarray(tot_E),
+                total_T_axis=T_axis, total_T=np.ascontiguousarray(tot_T), total_S=tot_S,
+                beta_partitions=parts("beta_partitions", "E_T"),
+                alpha_partitions=parts("alpha_partitions", "beta_T"),
+                beta_cutoff=float(t.get("beta_cutoff")),
+                alpha_cutoff=float(t.get("alpha_cutoff")))

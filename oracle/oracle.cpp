@@ -77,6 +77,7 @@ enum ErrBits : int {
   kErrResample = 2,
   kErrSteps = 4,
   kErrAlphaRange = 8,
+  kErrOverflow = 16,  // a 65th child of one history (site keys hold 64)
   kErrReaction = 32
 };
 constexpr int kMaxReactionResamples = 1000;
@@ -639,6 +640,10 @@ struct Ctx {
 // parent's next raw draw, taken after the child's direction (and energy).
 void BankSecondary(Ctx& c, Particle& p, const Pt& dir, double E) {
   const uint32_t seed = uint32_t(p.rng());
+  if (c.keig && c.child_count >= 64) {  // nu-bar < 63 is checked at orc_create
+    c.acc->err |= kErrOverflow;
+    return;
+  }
   if (c.keig) {
     mmc_site s{};
     s.x = p.pos.x;
@@ -1249,6 +1254,10 @@ int orc_create(const mmc_world_desc* wd, const mmc_tally_desc* td,
       n.fission_xs = vec(d.fission_xs);
       n.nubar = vec(d.nubar);
       n.has_fission = d.fission_xs != nullptr;
+      // site keys hold 64 children per history: nu-bar below 63
+      if (n.has_fission)
+        for (double v : n.nubar)
+          if (!(v >= 0 && v < 63.0)) { *out = ow; return Fail(ow, MMC_ERR_ARG, "nubar must lie in [0, 63)"); }
       n.has_scatter = d.scatter != nullptr;
       // TwoDimensional column construction and NormalizedTwoDimensional
       // (Multigroup.cpp:124-186); CreateScatterXS column sums (:201-210).
@@ -1304,6 +1313,10 @@ int orc_create(const mmc_world_desc* wd, const mmc_tally_desc* td,
         rx.xs = MakeTable(r.xs);
         rx.temperature = r.temperature;
         rx.nubar = MakeTable(r.nubar);
+        if (r.kind == MMC_RX_FISSION)
+          for (int i = 0; i < r.nubar.n; i++)
+            if (!(r.nubar.y[i] >= 0 && r.nubar.y[i] < 63.0))
+              { *out = ow; return Fail(ow, MMC_ERR_ARG, "nubar must lie in [0, 63)"); }
         rx.chi_kind = r.chi_kind;
         rx.watt_a = r.watt_a;
         rx.watt_b = r.watt_b;
@@ -1506,6 +1519,7 @@ static int ErrToStatus(orc_world* ow, int err) {
                 "Number of thermal scattering resamples exceeded limit: 10");
   if (err & kErrSteps) return Fail(ow, MMC_ERR_STEPS, "history step limit exceeded");
   if (err & kErrReaction) return Fail(ow, MMC_ERR_ARG, "no reaction selected after 1000 resamples");
+  if (err & kErrOverflow) return Fail(ow, MMC_ERR_OVERFLOW, "bank, queue or record overflow");
   return Fail(ow, MMC_ERR_ARG, "alpha sampling outside the beta grid");
 }
 

@@ -13,7 +13,9 @@ import numpy as np
 from minimc_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# MMC_ORACLE_LIB selects another build of the same sources (the sanitizer
+# builds of the Makefile, loaded only by the sanitizer test's child process).
+_LIB_PATH = os.environ.get("MMC_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 

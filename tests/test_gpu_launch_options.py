@@ -2,7 +2,9 @@
 tail kernel, bank compaction and the readback interval must leave every
 k-eigenvalue cycle bit-identical (same events per particle in the same
 order, so the same random numbers; k sums and the fission bank are reduced
-in a fixed order)."""
+in a fixed order).  The FP64 tallies (S, S2 of every bin: flux, currents,
+sensitivities) are reduced in a fixed order too (per-history sums, then a
+fixed tree per bin), so they are bit-identical across schedules as well."""
 import numpy as np
 import pytest
 
@@ -28,8 +30,15 @@ def _cycles(pr, opts, n_cycles, cap):
     t.keig_init(pr.histories)
     out = [t.run_cycle(active=c >= 1) for c in range(n_cycles)]
     c = t.counters()
+    c["tallies"] = [t.tallies(e) for e in range(len(pr.estimators))]
     t.close()
     return out, c
+
+
+def _same_tallies(a, b, what):
+    for e, ((S, S2), (T, T2)) in enumerate(zip(a, b)):
+        assert np.array_equal(S, T), (what, e, np.flatnonzero(S != T)[:5])
+        assert np.array_equal(S2, T2), (what, e, np.flatnonzero(S2 != T2)[:5])
 
 
 @pytest.mark.parametrize("case", ["multigroup_slab", "multizone"])
@@ -47,6 +56,7 @@ def test_schedules_give_identical_cycles(case):
                 assert a[k] == b[k], (case, opts, k, a[k], b[k])
         for k in ("collisions", "crossings", "advances", "tsl_collisions", "fission_sites"):
             assert c[k] == ref_c[k], (case, opts, k)
+        _same_tallies(ref_c["tallies"], c["tallies"], (case, opts))
     # the early-tail schedule really ran histories in the tail kernel
     assert runs[1][1]["tail_particles"] > 0
     assert ref_c["tail_particles"] == 0
@@ -75,3 +85,44 @@ def test_surrogate_schedules_give_identical_cycles():
             for k in KEYS:
                 assert a[k] == b[k], (k, a[k], b[k])
         assert c["tsl_fallbacks"] == rc["tsl_fallbacks"]
+
+
+def _ce_slab_with_sensitivity_and_flux():
+    """The continuous-energy slab (reduced grid) with a total cross-section
+    perturbation, the sensitivity of both leakage tallies to it, and a flux
+    mesh: every kind of fixed-source score."""
+    pr = configs.c2_slab(n_points=6000, n_res=20)
+    pr.perturbations.append(dict(name="total", nuclide=0))
+    for e in (0, 1):
+        pr.estimators.append(dict(pr.estimators[e], name="sens%d" % e, sensitivity=1,
+                                  parent=e))
+    pr.estimators.append(dict(kind=abi.MMC_EST_FLUX_MESH, name="flux",
+                              mesh=dict(n=[40, 1, 1], lo=[-20.0, -1e30, -1e30],
+                                        hi=[20.0, 1e30, 1e30])))
+    return pr
+
+
+@pytest.mark.parametrize("tracking", [abi.MMC_SURFACE_TRACKING, abi.MMC_CELL_DELTA_TRACKING])
+def test_fixed_source_tallies_bit_identical_across_schedules(tracking):
+    """Fixed source: the same histories under different schedules (tail,
+    compaction, readback interval, bank size and hence chunking of the
+    event loop within one chunk) give bit-identical S and S2."""
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = _ce_slab_with_sensitivity_and_flux()
+    n = 6000
+    res = []
+    for opts in SCHEDULES:
+        t = Tracker(pr, tracking=tracking, max_in_flight=1 << 13, **opts)
+        t.run_fixed_source(n)
+        res.append(([t.tallies(e) for e in range(len(pr.estimators))], t.counters()))
+        t.close()
+    ref, rc = res[0]
+    assert rc["histories"] == n
+    assert sum(S.sum() for S, _ in ref[:2]) > 0.05 * n  # some histories leak
+    assert ref[4][0].sum() > 0                           # flux was scored
+    assert np.any(ref[2][0] != 0)                         # sensitivities were scored
+    for (tal, c), opts in zip(res[1:], SCHEDULES[1:]):
+        _same_tallies(ref, tal, opts)
+        for k in ("collisions", "crossings", "advances"):
+            assert c[k] == rc[k], (opts, k)
