@@ -30,6 +30,7 @@ const DevWorld& MmcWorld(mmc_ctx* ctx);
 int MmcFail(mmc_ctx* ctx, int code, const char* msg);
 int MmcDevice(mmc_ctx* ctx);
 int MmcLaunchCheck(mmc_ctx* ctx, const char* where);
+int MmcTslCount(mmc_ctx* ctx);
 
 // Waits for the context's stream after a batch of launches; a launch that
 // failed at configuration time or a fault is an error.
@@ -949,6 +950,7 @@ static int Repack(mmc_ctx* ctx) {
 extern "C" int mmc_mlp_set(mmc_ctx* ctx, int tsl, const mmc_mlp_desc* d, const float* params,
                            uint64_t n_params) {
   if (!ctx || !d || d->n_layers < 1 || d->n_layers > kMlpMaxLayers) return MMC_ERR_ARG;
+  if (tsl < 0 || tsl >= MmcTslCount(ctx)) return MmcFail(ctx, MMC_ERR_ARG, "no such tsl table");
   if (hipSetDevice(MmcDevice(ctx)) != hipSuccess) return MMC_ERR_HIP;
   MlpState& m = MmcMlp(ctx);
   MlpFree(m);
@@ -1033,7 +1035,8 @@ extern "C" int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y)
 }
 
 extern "C" int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y) {
-  if (!ctx || !x || !y || tsl < 0) return MMC_ERR_ARG;
+  if (!ctx || !x || !y) return MMC_ERR_ARG;
+  if (tsl < 0 || tsl >= MmcTslCount(ctx)) return MmcFail(ctx, MMC_ERR_ARG, "no such tsl table");
   if (n == 0) return MMC_OK;
   if (n > 0xffffffffull) return MMC_ERR_ARG;
   hipLaunchKernelGGL(KPodMap, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, MmcStream(ctx),
@@ -1199,7 +1202,8 @@ __global__ void KSurrogateFinish(DevWorld W, MlpState m, int tsl, uint32_t n, co
 extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
                                         const double* T, uint32_t* rng, double* alpha,
                                         double* beta) {
-  if (!ctx || !E || !T || !rng || !alpha || !beta || tsl < 0) return MMC_ERR_ARG;
+  if (!ctx || !E || !T || !rng || !alpha || !beta) return MMC_ERR_ARG;
+  if (tsl < 0 || tsl >= MmcTslCount(ctx)) return MmcFail(ctx, MMC_ERR_ARG, "no such tsl table");
   MlpState& m = MmcMlp(ctx);
   if (!m.ready) return MmcFail(ctx, MMC_ERR_UNSUPPORTED, "no surrogate network set");
   if (n == 0) return MMC_OK;

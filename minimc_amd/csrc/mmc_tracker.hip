@@ -49,6 +49,7 @@ struct mmc_ctx {
   std::vector<void*> allocs;
   uint32_t* host_rb = nullptr;  // pinned copy of the ring and misc (one read per batch)
   bool has_tsl = false;
+  int n_tsl = 0;               // thermal-scattering tables of the World
   bool fixed_fission = false;  // fixed-source problem with fission children
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
@@ -572,6 +573,7 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       for (int k = 0; k < ce[n].rx_count; k++)
         if (rx[ce[n].rx_begin + k].tsl >= 0) tsl[rx[ce[n].rx_begin + k].tsl].awr = ce[n].awr;
     ctx->has_tsl = !tsl.empty();
+    ctx->n_tsl = int(tsl.size());
     if (!F.grids.empty()) ctx->n_hash0 = F.grids[0].n_hash;
   }
   // estimators
@@ -1909,8 +1911,8 @@ __global__ void __launch_bounds__(256)
 extern "C" int mmc_tsl_sample_pod(mmc_ctx* ctx, int tsl, uint64_t n, const double* E,
                                   const double* T, uint32_t* rng, double* alpha,
                                   double* beta) {
-  if (!ctx || !ctx->has_tsl || tsl < 0 || !E || !T || !rng || !alpha || !beta)
-    return MMC_ERR_ARG;
+  if (!ctx || !E || !T || !rng || !alpha || !beta) return MMC_ERR_ARG;
+  if (tsl < 0 || tsl >= ctx->n_tsl) return Fail(ctx, MMC_ERR_ARG, "no such tsl table");
   if (n == 0) return MMC_OK;
   if (n > 0xffffffffull) return Fail(ctx, MMC_ERR_ARG, "batch too large");
   HIPCHK(hipSetDevice(ctx->device));
@@ -1961,3 +1963,4 @@ MlpState& MmcMlp(mmc_ctx* ctx) { return ctx->mlp; }
 const DevWorld& MmcWorld(mmc_ctx* ctx) { return ctx->W; }
 int MmcFail(mmc_ctx* ctx, int code, const char* msg) { return Fail(ctx, code, msg); }
 int MmcDevice(mmc_ctx* ctx) { return ctx->device; }
+int MmcTslCount(mmc_ctx* ctx) { return ctx->n_tsl; }

@@ -90,3 +90,18 @@ def test_broomstick_histograms_match_oracle():
         assert np.all(np.abs(go - gt) <= 4 * np.sqrt(go + gt + 1) + 3)
     c = t.counters()
     assert c["tsl_collisions"] > 0.5 * n
+
+
+def test_tsl_index_out_of_range_is_an_error(pair):
+    """A thermal-scattering table index beyond the World's tables is refused
+    before any kernel reads the table descriptors."""
+    from minimc_amd.tracker import MmcError
+    pr, _, t = pair
+    n = 16
+    E, T = t.device_array(np.full(n, 1e-7)), t.device_array(np.full(n, 600.0))
+    r = t.device_array(np.arange(1, n + 1, dtype=np.uint32))
+    a = t.device_array(shape=n, dtype=np.float64)
+    b = t.device_array(shape=n, dtype=np.float64)
+    for bad in (-1, len(pr.tsl)):
+        with pytest.raises(MmcError, match="no such tsl table"):
+            t.tsl_sample_pod(E, T, r, a, b, tsl=bad)
