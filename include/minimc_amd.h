@@ -280,7 +280,9 @@ typedef struct mmc_options {
   int32_t device;       /* HIP device ordinal                           */
   int32_t block;        /* threads per workgroup (0 = default 256)      */
   uint64_t max_in_flight; /* particles resident per chunk (0 = default)  */
-  int32_t graph_iters;  /* event iterations per captured graph (0 = 8)   */
+  int32_t graph_iters;  /* event iterations per host read of the queue
+                           lengths, and per captured graph when graphs is
+                           on (0 = 8)                                     */
   int32_t tail_threshold; /* live particles below which the fused tail
                              kernel finishes a batch (0 = default,
                              -1 = never)                                */
@@ -290,6 +292,16 @@ typedef struct mmc_options {
                              fill less than half of the slots in use (at a
                              read of the queue lengths), k > 0 = before
                              every k-th iteration, -1 = never             */
+  int32_t graphs;       /* 1: k-eigenvalue cycles replay each batch of
+                           graph_iters iterations (and its read of the
+                           queue lengths) as a captured HIP graph, one per
+                           ring phase and launch shape; 0: launch the
+                           kernels one by one                             */
+  int32_t tail_block;   /* threads per block of the tail kernel: 256 or 512
+                           (0 = 256)                                      */
+  int32_t tsl_block;    /* threads per block of the POD thermal-scattering
+                           kernel: 256 or 384 (0 = 256)                   */
+  int32_t pad_o;
 } mmc_options;
 /* Reference behaviours that are reproduced only on request.
  * MMC_QUIRK_NUDGE_CROSSING: after a surface crossing the new cell is the
@@ -360,6 +372,8 @@ typedef struct mmc_counters {
   /* events run inside the tail kernel (also counted above) */
   uint64_t tail_advances, tail_collisions, tail_tsl, tail_crossings;
   uint64_t mlp_rows;     /* thermal collisions sampled by the network     */
+  uint64_t graph_launches, graph_captures; /* batches replayed as graphs,
+                                              and graphs captured         */
 } mmc_counters;
 
 /* ---- k-eigenvalue cycle statistics -------------------------------------- */
@@ -491,10 +505,11 @@ int mmc_mlp_get(mmc_ctx* ctx, float* params, uint64_t n_params);
  * FP32 accumulation inside (widths 4-256-256-256-2 only). */
 int mmc_mlp_infer(mmc_ctx* ctx, uint64_t n, const float* x, float* y);
 /* The deterministic POD transport map the network is trained to imitate:
- * the POD inverse CDFs of beta (level u1) and then alpha (level u2), with
- * linear instead of stochastic interpolation between the tabulated
- * energies and betas.  Device x [n x 4] raw inputs -> y [n x 2] raw
- * outputs in the network's output parametrisation. */
+ * the quantile of beta at level u1 and then of alpha at level u2 under the
+ * POD sampler's own laws, i.e. the mixtures over the two bracketing
+ * tabulated energies (betas) that its stochastic interpolation draws from,
+ * truncated where it rejects.  Device x [n x 4] raw inputs -> y [n x 2]
+ * raw outputs in the network's output parametrisation. */
 int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y);
 /* Surrogate sampling of (alpha, beta) for device batches with fallback. */
 int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n,

@@ -157,7 +157,9 @@ class Options(C.Structure):
                 ("device", C.c_int32), ("block", C.c_int32),
                 ("max_in_flight", C.c_uint64), ("graph_iters", C.c_int32),
                 ("tail_threshold", C.c_int32), ("max_steps", C.c_uint64),
-                ("quirks", C.c_int32), ("compact_interval", C.c_int32)]
+                ("quirks", C.c_int32), ("compact_interval", C.c_int32),
+                ("graphs", C.c_int32), ("tail_block", C.c_int32),
+                ("tsl_block", C.c_int32), ("pad_o", C.c_int32)]
 
 
 class ParticleSoA(C.Structure):
@@ -174,7 +176,8 @@ class Counters(C.Structure):
         "tsl_fallbacks", "crossings", "virtual_collisions", "fission_sites",
         "iterations", "tail_particles")] + [("transport_seconds", C.c_double)] + [
         (n, C.c_uint64) for n in ("tail_advances", "tail_collisions", "tail_tsl",
-                                  "tail_crossings", "mlp_rows")]
+                                  "tail_crossings", "mlp_rows", "graph_launches",
+                                  "graph_captures")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

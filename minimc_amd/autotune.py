@@ -123,6 +123,18 @@ def tune(problem, n=None, cycles=3, tsl_mode=abi.MMC_TSL_POD, prepare=None,
                 opts["tail_threshold"], opts["graph_iters"], cp, s * 1e3))
         if s < best_s * (1 - MARGIN):
             best, best_s = dict(opts), s
+    # batches replayed as captured graphs (k-eigenvalue; compaction every
+    # iteration, at readbacks or never)
+    if problem.problem == "keigenvalue" and best.get("compact_interval", 0) <= 1:
+        opts = dict(best, graphs=1)
+        s = _time_run(problem, opts, n, cycles, tsl_mode, prepare)
+        table.append((dict(opts), s))
+        if log:
+            log("tail %6d iters %2d compact %2d graphs: %.2f ms" % (
+                opts["tail_threshold"], opts["graph_iters"],
+                opts.get("compact_interval", 0), s * 1e3))
+        if s < best_s * (1 - MARGIN):
+            best, best_s = dict(opts), s
     if use_cache:
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         cache[key] = {"best": best, "table": table}

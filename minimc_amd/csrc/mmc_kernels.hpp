@@ -1024,8 +1024,10 @@ __device__ __forceinline__ bool TslPodEvent(const DevWorld& W, const RunState& R
 // dynamic LDS and only as many blocks as are resident at once).
 // (The descriptor block stays in HBM here: a sample reads two descriptors,
 // and the sampler's tables alone keep the LDS at two blocks per CU.)
-template <bool kLds>
-__global__ void __launch_bounds__(256) KTslPod(DevWorld W, RunState R, int iter, int qin) {
+// kThreads: threads per block (the launch option tsl_block): 384 fits three
+// waves per SIMD in two blocks of a CU's LDS.
+template <bool kLds, int kThreads = 256>
+__global__ void __launch_bounds__(kThreads) KTslPod(DevWorld W, RunState R, int iter, int qin) {
   extern __shared__ double lds_hot[];
   const uint32_t n = *Ring(R, iter, kT);
   if (blockIdx.x * blockDim.x >= n) return;  // whole block idle: skip the staging
@@ -1089,8 +1091,12 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 // history ends takes the next queued particle.  Thermal scatterings use the
 // POD sampler here.  Used only where a collision cannot start a new particle
 // (k-eigenvalue, or no fission).
-template <bool kLds>
-__global__ void __launch_bounds__(256) KTail(DevWorld Wg, RunState R, int iter, int qin) {
+// kThreads: threads per block (the launch option tail_block).  The tail
+// stages more LDS than two blocks of a CU can hold, so its occupancy is one
+// block per CU: 512 threads make that two waves per SIMD (and cap the
+// registers at 256 per lane, arch and acc together).
+template <bool kLds, int kThreads = 256>
+__global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCol = 1, kTsl = 2, kDone = 3 };
   extern __shared__ int lds_hash[];
   uint32_t n1;

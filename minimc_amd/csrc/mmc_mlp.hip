@@ -86,7 +86,7 @@ __device__ __forceinline__ float SiluFast(float z) {
 
 void MlpFree(MlpState& m) {
   void* ps[] = {m.params, m.wbf16, m.adam_m, m.adam_v, m.grads, m.act, m.loss,
-                m.xin, m.yout, m.fb, m.packed};
+                m.xin, m.yout, m.berr, m.fb, m.packed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   m = MlpState{};
@@ -701,6 +701,8 @@ __global__ void __launch_bounds__(256) KTslPodFallback(DevWorld W, RunState R, i
   }
 }
 __global__ void KClearFallback(RunState R) { R.misc[kMiscFallback] = 0; }
+// Clears one error word (a kernel, not a memset: no memset node in graphs).
+__global__ void KClearWord(uint32_t* w) { *w = 0u; }
 
 static MlpDev DevView(const MlpState& m) {
   MlpDev d;
@@ -1209,31 +1211,36 @@ extern "C" int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n, const
   if (n == 0) return MMC_OK;
   if (n > 0xffffffffull) return MMC_ERR_ARG;
   hipStream_t s = MmcStream(ctx);
-  float *xin = nullptr, *yout = nullptr;
-  uint32_t* err = nullptr;
-  if (hipMalloc(&xin, sizeof(float) * 4 * n) != hipSuccess ||
-      hipMalloc(&yout, sizeof(float) * 2 * n) != hipSuccess ||
-      hipMalloc(&err, sizeof(uint32_t)) != hipSuccess)
-    return MMC_ERR_HIP;
-  hipMemsetAsync(err, 0, sizeof(uint32_t), s);
+  // the network's inputs and outputs and the error word, kept by the context
+  // and grown to the largest batch seen
+  if (m.batch_cap < n) {
+    if (hipStreamSynchronize(s) != hipSuccess) return MMC_ERR_HIP;
+    for (void* p : {static_cast<void*>(m.xin), static_cast<void*>(m.yout),
+                    static_cast<void*>(m.berr)})
+      if (p) (void)hipFree(p);
+    m.xin = m.yout = nullptr;
+    m.berr = nullptr;
+    m.batch_cap = 0;
+    if (hipMalloc(&m.xin, sizeof(float) * 4 * n) != hipSuccess ||
+        hipMalloc(&m.yout, sizeof(float) * 2 * n) != hipSuccess ||
+        hipMalloc(&m.berr, sizeof(uint32_t)) != hipSuccess)
+      return MmcFail(ctx, MMC_ERR_HIP, "surrogate staging allocation failed");
+    m.batch_cap = n;
+  }
+  float* xin = m.xin;
+  float* yout = m.yout;
+  uint32_t* err = m.berr;
+  hipLaunchKernelGGL(KClearWord, dim3(1), dim3(1), 0, s, err);
   const uint32_t nb = uint32_t((n + 255) / 256);
   hipLaunchKernelGGL(KSurrogateBatch, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
                      uint32_t(n), E, T, rng, xin);
-  if (InferChunked(ctx, m, xin, yout, n)) {
-    (void)hipFree(xin);
-    (void)hipFree(yout);
-    (void)hipFree(err);
-    return MMC_ERR_HIP;
-  }
+  if (InferChunked(ctx, m, xin, yout, n)) return MMC_ERR_HIP;
   hipLaunchKernelGGL(KSurrogateFinish, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
                      uint32_t(n), E, T, rng, yout, alpha, beta, err);
   uint32_t herr = 0;
-  hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-  const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
-  (void)hipFree(xin);
-  (void)hipFree(yout);
-  (void)hipFree(err);
-  if (!ok) return MMC_ERR_HIP;
+  if (hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "error word readback failed");
+  if (const int rc = SyncChecked(ctx, "mmc_tsl_sample_surrogate")) return rc;
   if (herr) return MmcFail(ctx, MMC_ERR_RESAMPLE, "POD fallback failed");
   return MMC_OK;
 }

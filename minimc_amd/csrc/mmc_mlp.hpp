@@ -27,8 +27,10 @@ struct MlpState {
   size_t act_cap = 0;
   float* loss = nullptr;
   long adam_t = 0;
-  float* xin = nullptr;           // unused
-  float* yout = nullptr;          // unused
+  float* xin = nullptr;           // batch sampler: network inputs [batch_cap x 4]
+  float* yout = nullptr;          // and outputs [batch_cap x 2]
+  uint32_t* berr = nullptr;       // and its error word
+  uint64_t batch_cap = 0;
   uint32_t* fb = nullptr;         // parked fallback particles [cap]
   uint32_t stage_cap = 0;         // rows of the tracker buffers
 };

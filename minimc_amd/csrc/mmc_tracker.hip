@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -54,6 +56,8 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
+  int tail_block = 256;        // threads per block of KTail (256 or 512)
+  int tsl_block = 256;         // threads per block of KTslPod (256 or 384)
   size_t hot_lds = 0;          // LDS bytes of KTslPod<true> (0: it does not stage)
   uint32_t tsl_grid = 1024;    // grid cap of KTslPod when it stages the hot block
   size_t ev_lds = 0;           // LDS bytes of KAdvance / KCollide
@@ -94,6 +98,17 @@ struct mmc_ctx {
   size_t ev_used = 0;
   double k_ms[MMC_K_N] = {};
   uint64_t k_launches[MMC_K_N] = {};
+  // Captured batches of event iterations (RunEvents with opt.graphs), keyed
+  // by everything the captured launch arguments depend on: ring phase, queue
+  // parity, grid, the bank arrays in use, the record buffers, the network.
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> kinds;  // profiling brackets recorded by the graph
+    size_t ev_used = 0;
+    int swaps = 0;           // bank swaps the batch makes
+  };
+  std::map<std::array<uint64_t, 8>, GraphEntry> graphs;
+  uint64_t graph_captures = 0, graph_launches = 0;
 };
 
 template <class T>
@@ -717,7 +732,9 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       {StaticLds(reinterpret_cast<const void*>(KAdvance<false, false>)),
        StaticLds(reinterpret_cast<const void*>(KAdvance<false, true>)),
        StaticLds(reinterpret_cast<const void*>(KCollide<false>))});
-  const size_t tail_static = StaticLds(reinterpret_cast<const void*>(KTail<true>));
+  const size_t tail_static =
+      std::max(StaticLds(reinterpret_cast<const void*>(KTail<true>)),
+               StaticLds(reinterpret_cast<const void*>(KTail<true, 512>)));
   if (ctx->hot_lds &&
       ctx->hot_lds + StaticLds(reinterpret_cast<const void*>(KTslPod<true>)) > kLdsPerBlock)
     ctx->hot_lds = 0;
@@ -857,6 +874,16 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     return MMC_ERR_NODEV;
   }
   ctx->device = ctx->opt.device;
+  if (ctx->opt.tail_block != 0 && ctx->opt.tail_block != 256 && ctx->opt.tail_block != 512) {
+    delete ctx;
+    return MMC_ERR_ARG;
+  }
+  if (ctx->opt.tsl_block != 0 && ctx->opt.tsl_block != 256 && ctx->opt.tsl_block != 384) {
+    delete ctx;
+    return MMC_ERR_ARG;
+  }
+  ctx->tail_block = ctx->opt.tail_block ? ctx->opt.tail_block : 256;
+  ctx->tsl_block = ctx->opt.tsl_block ? ctx->opt.tsl_block : 256;
   if (hipSetDevice(ctx->device) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
@@ -895,17 +922,26 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     // the tail kernel's lanes take the next particle when one ends: launch
     // only the blocks that are resident at once
     int occ_t = 0;
+    const int tb = ctx->tail_block;
     const hipError_t et =
-        ctx->tail_lds
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<true>, kBlock,
-                                                           ctx->tail_lds)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, KTail<false>, kBlock, 0);
+        tb == 512 ? (ctx->tail_lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                         &occ_t, KTail<true, 512>, tb, ctx->tail_lds)
+                                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                         &occ_t, KTail<false, 512>, tb, 0))
+                  : (ctx->tail_lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                         &occ_t, KTail<true>, tb, ctx->tail_lds)
+                                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                         &occ_t, KTail<false>, tb, 0));
     if (et != hipSuccess) occ_t = 1;
     ctx->tail_grid = uint32_t(std::max(1, occ_t)) * uint32_t(n_cu);
     // KTslPod with the hot block in LDS: resident blocks only
     int occ_s = 0;
-    if (ctx->hot_lds && hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                            &occ_s, KTslPod<true>, kBlock, ctx->hot_lds) != hipSuccess)
+    if (ctx->hot_lds &&
+        (ctx->tsl_block == 384
+             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, KTslPod<true, 384>, 384,
+                                                            ctx->hot_lds)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, KTslPod<true>, kBlock,
+                                                            ctx->hot_lds)) != hipSuccess)
       occ_s = 1;
     ctx->tsl_grid = uint32_t(std::max(1, occ_s)) * uint32_t(n_cu);
   }
@@ -920,6 +956,7 @@ extern "C" void mmc_destroy(mmc_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (auto& g : ctx->graphs) hipGraphExecDestroy(g.second.exec);
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void* p : ctx->allocs) hipFree(p);
   if (ctx->R.rec_key) hipFree(ctx->R.rec_key);
@@ -1130,7 +1167,10 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid, bool 
       if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
     } else {
       ProfBegin(ctx, MMC_K_TSL);
-      if (ctx->hot_lds)
+      if (ctx->hot_lds && ctx->tsl_block == 384)
+        hipLaunchKernelGGL((KTslPod<true, 384>), dim3(std::min(grid, ctx->tsl_grid)), dim3(384),
+                           ctx->hot_lds, ctx->stream, ctx->W, ctx->R, iter, qin);
+      else if (ctx->hot_lds)
         hipLaunchKernelGGL(KTslPod<true>, dim3(std::min(grid, ctx->tsl_grid)), dim3(kBlock),
                            ctx->hot_lds, ctx->stream, ctx->W, ctx->R, iter, qin);
       else
@@ -1165,6 +1205,102 @@ extern "C" int mmc_get_kernel_times(mmc_ctx* ctx, double* total_ms, uint64_t* la
 }
 
 
+// One batch of K event iterations from iteration `iter`, then the POD
+// sampler for the collisions the surrogate parked, then the copy of the ring
+// and counters into pinned host memory (the caller synchronises).  `swaps`
+// counts the bank swaps of the batch's compactions.
+static int LaunchBatch(mmc_ctx* ctx, int& iter, int& qin, uint32_t grid, int K,
+                       bool compact_ok, int every, bool surrogate, int& swaps) {
+  for (int k = 0; k < K; k++) {
+    // compaction every iteration is done by the advance kernel itself
+    const bool move = compact_ok && every == 1 && iter > 0;
+    if (compact_ok && every > 1 && iter > 0 && iter % every == 0) {
+      hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
+                         ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr,
+                         ctx->W.n_pert);
+      SwapBanks(ctx);
+      swaps++;
+    }
+    const int rc =
+        LaunchIteration(ctx, iter, qin, grid, move, surrogate ? ctx->mlp.fb : nullptr);
+    if (rc) return rc;
+    if (move) swaps++;
+    iter++;
+    qin ^= 1;
+  }
+  if (surrogate) {  // parked fallback particles rejoin before the count is read
+    ProfBegin(ctx, MMC_K_TSL);
+    const int rc =
+        MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->hot_lds, ctx->stream);
+    ProfEnd(ctx);
+    if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
+  }
+  if (const int lc = LaunchCheck(ctx, "event kernels")) return lc;
+  HIPCHK(hipMemcpyAsync(ctx->host_rb, ctx->R.ring,
+                        sizeof(uint32_t) * (kRing * kRingStride + 16), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  return MMC_OK;
+}
+
+// The grid of a captured batch: the next power of two, so that a few graphs
+// serve every live count (the kernels grid-stride over their queues and
+// blocks without work return at once).
+static uint32_t GraphGrid(uint32_t grid) {
+  uint32_t g = 1;
+  while (g < grid) g <<= 1;
+  return std::min<uint32_t>(g, 4096u);
+}
+
+// LaunchBatch as a HIP graph: captured the first time its key is seen, then
+// replayed.  A replay makes the host-side changes the capture made (the bank
+// swaps, iteration and queue parity) without launching anything itself.
+static int GraphBatch(mmc_ctx* ctx, int& iter, int& qin, uint32_t grid, int K,
+                      bool compact_ok, int every, bool surrogate) {
+  const std::array<uint64_t, 8> key = {
+      uint64_t(iter % kRing), uint64_t(qin), grid,
+      uint64_t(reinterpret_cast<uintptr_t>(ctx->R.B.x)),
+      uint64_t(reinterpret_cast<uintptr_t>(ctx->R.rec_key)), ctx->R.rec_cap,
+      uint64_t(reinterpret_cast<uintptr_t>(ctx->mlp.packed)) ^
+          (uint64_t(reinterpret_cast<uintptr_t>(ctx->mlp.fb)) << 1),
+      uint64_t(surrogate) | uint64_t(ctx->prof) << 1 | uint64_t(every + 1) << 2};
+  auto it = ctx->graphs.find(key);
+  if (it == ctx->graphs.end()) {
+    mmc_ctx::GraphEntry ent;
+    const size_t kinds0 = ctx->ev_kind.size();
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int swaps = 0;
+    const int rc = LaunchBatch(ctx, iter, qin, grid, K, compact_ok, every, surrogate, swaps);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+      if (g) hipGraphDestroy(g);
+      return rc ? rc : Fail(ctx, MMC_ERR_HIP, std::string("graph capture: ") +
+                                                  hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&ent.exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess)
+      return Fail(ctx, MMC_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    ent.kinds.assign(ctx->ev_kind.begin() + kinds0, ctx->ev_kind.end());
+    ent.ev_used = ctx->ev_used;
+    ent.swaps = swaps;
+    it = ctx->graphs.emplace(key, std::move(ent)).first;
+    ctx->graph_captures++;
+  } else {
+    for (int s = 0; s < it->second.swaps; s++) SwapBanks(ctx);
+    iter += K;
+    qin ^= K & 1;
+    if (ctx->prof) {
+      // the graph records the same events, from the start of the pool
+      ctx->ev_kind = it->second.kinds;
+      ctx->ev_used = it->second.ev_used;
+    }
+  }
+  HIPCHK(hipGraphLaunch(it->second.exec, ctx->stream));
+  ctx->graph_launches++;
+  return MMC_OK;
+}
+
 // Runs event iterations until no particle is alive.  The live-queue length
 // is read back every `K` iterations; in between, kernels read queue lengths
 // from device memory and grid-stride over them.
@@ -1194,17 +1330,30 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   // only where no collision allocates new slots
   const int every = ctx->opt.compact_interval;
   const bool compact_ok = every >= 0 && (ctx->R.keig || !ctx->fixed_fission);
+  // Captured batches: k-eigenvalue cycles (the launch arguments repeat from
+  // cycle to cycle), with compaction every iteration, at readbacks or never
+  // (KCompact every k > 1 iterations would tie a batch to iter mod k).
+  const bool graphs = ctx->opt.graphs > 0 && ctx->R.keig && every <= 1;
   uint32_t in_use = n0;
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
       ProfBegin(ctx, MMC_K_TAIL);
-      const dim3 tgrid(std::min(Blocks(bound), ctx->tail_grid));
-      if (ctx->tail_lds)
+      const uint32_t tb = uint32_t(ctx->tail_block);
+      const dim3 tgrid(std::min(uint32_t((bound + tb - 1) / tb), ctx->tail_grid));
+      if (tb == 512) {
+        if (ctx->tail_lds)
+          hipLaunchKernelGGL((KTail<true, 512>), tgrid, dim3(tb), ctx->tail_lds, ctx->stream,
+                             ctx->W, ctx->R, iter, qin);
+        else
+          hipLaunchKernelGGL((KTail<false, 512>), tgrid, dim3(tb), 0, ctx->stream, ctx->W,
+                             ctx->R, iter, qin);
+      } else if (ctx->tail_lds) {
         hipLaunchKernelGGL(KTail<true>, tgrid, dim3(kBlock), ctx->tail_lds, ctx->stream,
                            ctx->W, ctx->R, iter, qin);
-      else
+      } else {
         hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
                            iter, qin);
+      }
       ProfEnd(ctx);
       if (const int lc = LaunchCheck(ctx, "tail kernel")) return lc;
       HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, kMiscBytes, hipMemcpyDeviceToHost,
@@ -1215,33 +1364,16 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       if (rc) return rc;
       break;
     }
-    const uint32_t grid = Blocks(bound);
-    for (int k = 0; k < K; k++) {
-      // compaction every iteration is done by the advance kernel itself
-      const bool move = compact_ok && every == 1 && iter > 0;
-      if (compact_ok && every > 1 && iter > 0 && iter % every == 0) {
-        hipLaunchKernelGGL(KCompact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->R,
-                           ctx->alt, iter, qin, surrogate ? ctx->mlp.fb : nullptr,
-                           ctx->W.n_pert);
-        SwapBanks(ctx);
-      }
-      const int rc =
-          LaunchIteration(ctx, iter, qin, grid, move, surrogate ? ctx->mlp.fb : nullptr);
+    if (graphs && iter > 0) {
+      const int rc = GraphBatch(ctx, iter, qin, GraphGrid(Blocks(bound)), K, compact_ok,
+                                every, surrogate);
       if (rc) return rc;
-      iter++;
-      qin ^= 1;
-    }
-    if (surrogate) {  // parked fallback particles rejoin before the count is read
-      ProfBegin(ctx, MMC_K_TSL);
+    } else {
+      int swaps = 0;
       const int rc =
-          MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->hot_lds, ctx->stream);
-      ProfEnd(ctx);
-      if (rc) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
+          LaunchBatch(ctx, iter, qin, Blocks(bound), K, compact_ok, every, surrogate, swaps);
+      if (rc) return rc;
     }
-    if (const int lc = LaunchCheck(ctx, "event kernels")) return lc;
-    HIPCHK(hipMemcpyAsync(ring, ctx->R.ring,
-                          sizeof(uint32_t) * kRing * kRingStride + kMiscBytes,
-                          hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ProfCollect(ctx);
     ctx->cnt.iterations += K;
@@ -1289,6 +1421,8 @@ static int ReadEvents(mmc_ctx* ctx) {
   ctx->cnt.tail_tsl = ctx->ev_host[kEvTailTsl];
   ctx->cnt.tail_crossings = ctx->ev_host[kEvTailCross];
   ctx->cnt.mlp_rows = ctx->ev_host[kEvMlp];
+  ctx->cnt.graph_launches = ctx->graph_launches;
+  ctx->cnt.graph_captures = ctx->graph_captures;
   return MMC_OK;
 }
 

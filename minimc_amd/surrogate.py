@@ -146,16 +146,45 @@ class Surrogate:
         self._check(lib().mmc_mlp_infer(self.t.h, n, x_dev.data_ptr(), y.data_ptr()))
         return y
 
+    def bin_errors(self, n=1 << 20, seed=7919):
+        """Per (ln E, T) weight bin and output: the mean absolute error of the
+        network (BF16 inference) against the POD map on n fresh inputs,
+        relative to the spread of the target in that bin.  For a network
+        monotone in u this is the bin's Wasserstein-1 error of the marginal,
+        relative to its standard deviation."""
+        x = sample_inputs(n, seed)
+        xd = self.t.device_array(x)
+        y = self.pod_map(xd, n)
+        p = self.infer(xd, n)
+        yh, ph = y.numpy().astype(np.float64), p.numpy().astype(np.float64)
+        for a in (xd, y, p):
+            a.free()
+        b = weight_bins(x)
+        nb = WEIGHT_BINS[0] * WEIGHT_BINS[1]
+        cnt = np.maximum(np.bincount(b, minlength=nb), 1)
+        err = np.zeros((nb, 2))
+        for k in range(2):
+            m1 = np.bincount(b, yh[:, k], minlength=nb) / cnt
+            m2 = np.bincount(b, yh[:, k] ** 2, minlength=nb) / cnt
+            sd = np.sqrt(np.maximum(m2 - m1 * m1, 1e-24))
+            err[:, k] = np.bincount(b, np.abs(ph[:, k] - yh[:, k]), minlength=nb) / cnt / sd
+        return err
+
     def train(self, n_samples=1 << 26, batch=8192, epochs=3, lr=2e-3, seed=11,
-              chunk=1 << 24, log=None, weighted=True):
+              chunk=1 << 24, log=None, weighted=True, rounds=1, gamma=1.0, lr_floor=1e-5):
         """Generates n_samples POD targets on the device and trains (with the
-        per-(E, T) loss weights above unless weighted is False)."""
+        per-(E, T) loss weights above unless weighted is False).  With
+        rounds > 1 the epochs are split into rounds, and after each round but
+        the last the weight of every (ln E, T) bin is multiplied by
+        (its relative error / the mean relative error)^gamma (bin_errors):
+        the training effort moves to where the network is worst, which is
+        what a bound on the largest error needs."""
         n = n_samples
         x = self.t.device_array(shape=(n, 4), dtype=np.float32)
         y = self.t.device_array(shape=(n, 2), dtype=np.float32)
         w = self.t.device_array(shape=(n, 2), dtype=np.float32) if weighted else None
         # inputs in chunks (host RNG), targets from the POD map on the device
-        stats, wtab = [], None
+        stats, wtab, b0 = [], None, None
         for c0 in range(0, n, chunk):
             m = min(chunk, n - c0)
             xc = sample_inputs(m, seed + c0)
@@ -168,6 +197,7 @@ class Surrogate:
                 stats = (yc.mean(0), yc.std(0) + 1e-6)
                 if weighted:
                     wtab = weight_table(xc, yc)
+                    b0 = weight_bins(xc)
             if weighted:
                 wc = np.ascontiguousarray(wtab[weight_bins(xc)], dtype=np.float32)
                 lib().mmc_memcpy_h2d(self.t.h, w.ptr + c0 * 8, wc.ctypes.data, wc.nbytes)
@@ -177,14 +207,30 @@ class Surrogate:
         self._check(lib().mmc_mlp_normalize(self.t.h, n, x.data_ptr(), y.data_ptr()))
         steps_per_epoch = n // batch
         losses = []
+        per_round = max(1, epochs // max(rounds, 1))
         for ep in range(epochs):
+            if weighted and rounds > 1 and ep > 0 and ep % per_round == 0 and \
+                    ep // per_round < rounds:
+                err = self.bin_errors()
+                if log:
+                    log("round %d: largest bin error beta %.4f alpha %.4f" % (
+                        ep // per_round, err[:, 0].max(), err[:, 1].max()))
+                for k in range(2):
+                    wtab[:, k] *= (err[:, k] / err[:, k].mean()) ** gamma
+                    wtab[:, k] = np.clip(wtab[:, k], 0.05, 50.0)
+                    wtab[:, k] /= wtab[b0, k].mean()
+                for c0 in range(0, n, chunk):
+                    m = min(chunk, n - c0)
+                    wc = np.ascontiguousarray(wtab[weight_bins(sample_inputs(m, seed + c0))],
+                                              dtype=np.float32)
+                    lib().mmc_memcpy_h2d(self.t.h, w.ptr + c0 * 8, wc.ctypes.data, wc.nbytes)
             # cosine-decayed learning rate in 8 chunks per epoch
             for part in range(8):
                 frac = (ep * 8 + part) / (epochs * 8)
                 o = abi.TrainOpts()
                 o.batch = batch
                 o.steps = steps_per_epoch // 8
-                o.lr = float(lr * 0.5 * (1 + np.cos(np.pi * frac)) + 1e-5)
+                o.lr = float(lr * 0.5 * (1 + np.cos(np.pi * frac)) + lr_floor)
                 o.beta1, o.beta2, o.eps = 0.9, 0.999, 1e-8
                 o.offset = (part * (steps_per_epoch // 8)) * batch
                 o.weights = w.ptr if weighted else None

@@ -68,6 +68,12 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--rounds", type=int, default=1,
+                    help="training rounds; between rounds the loss weight of each "
+                         "(ln E, T) bin follows its relative error")
+    ap.add_argument("--gamma", type=float, default=1.0)
+    ap.add_argument("--lr-floor", type=float, default=1e-5)
+    ap.add_argument("--seed", type=int, default=11)
     args = ap.parse_args()
     pr = configs.tsl_box(configs.tsl_tables(), T=600.0)
     t = Tracker(pr, max_in_flight=1 << 16)
@@ -79,11 +85,13 @@ def main():
     else:
         t0 = time.perf_counter()
         losses = s.train(n_samples=1 << args.samples, batch=args.batch, epochs=args.epochs,
-                         lr=args.lr, log=lambda m: print(m, flush=True))
+                         lr=args.lr, log=lambda m: print(m, flush=True), rounds=args.rounds,
+                         gamma=args.gamma, lr_floor=args.lr_floor, seed=args.seed)
         dt = time.perf_counter() - t0
         steps = args.epochs * ((1 << args.samples) // args.batch)
         info = dict(train_seconds=dt, steps=steps, samples=1 << args.samples,
-                    epochs=args.epochs, batch=args.batch, lr=args.lr,
+                    epochs=args.epochs, batch=args.batch, lr=args.lr, rounds=args.rounds,
+                    gamma=args.gamma, lr_floor=args.lr_floor, seed=args.seed,
                     final_loss=losses[-1],
                     train_tflops=steps * args.batch * 3 * 2 * surrogate.n_params() / dt / 1e12)
         s.save(out, info=info)

@@ -19,6 +19,9 @@ SCHEDULES = [
     dict(tail_threshold=2048, compact_interval=1, graph_iters=8),  # compaction in KAdvance
     dict(tail_threshold=2048, compact_interval=2, graph_iters=8),  # KCompact
     dict(),                                                         # defaults
+    # batches replayed as captured graphs (k-eigenvalue only)
+    dict(tail_threshold=2048, compact_interval=1, graph_iters=8, graphs=1),
+    dict(tail_threshold=-1, compact_interval=0, graph_iters=4, graphs=1),
 ]
 KEYS = ("k_analog", "k_collision", "k_track", "k_absorption", "entropy", "n_sites")
 
@@ -60,6 +63,16 @@ def test_schedules_give_identical_cycles(case):
     # the early-tail schedule really ran histories in the tail kernel
     assert runs[1][1]["tail_particles"] > 0
     assert ref_c["tail_particles"] == 0
+    # the graph schedules replayed captured batches, from a few captures; with
+    # a tail threshold the small slab can reach the tail before its first
+    # captured batch, without the tail it cannot
+    for (_, c), opts in zip(runs, SCHEDULES):
+        if opts.get("graphs"):
+            assert c["graph_launches"] >= c["graph_captures"], (case, opts, c)
+            if opts["tail_threshold"] < 0 or case == "multizone":
+                assert c["graph_launches"] > c["graph_captures"] > 0, (case, opts, c)
+        else:
+            assert c["graph_launches"] == 0
 
 
 def test_surrogate_schedules_give_identical_cycles():
@@ -70,9 +83,9 @@ def test_surrogate_schedules_give_identical_cycles():
     from minimc_amd.tracker import Tracker
     pr = configs.c5_multizone(histories=20000)
     res = []
-    for cp in (-1, 1, 2):
+    for cp, gr in ((-1, 0), (1, 0), (2, 0), (1, 1)):
         t = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=1 << 16,
-                    compact_interval=cp, tail_threshold=2048)
+                    compact_interval=cp, tail_threshold=2048, graphs=gr)
         s, _ = surrogate.cached_network(t)  # the shipped network
         t.set_entropy_mesh(*pr.entropy_mesh)
         t.keig_init(pr.histories)
