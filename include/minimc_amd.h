@@ -515,6 +515,18 @@ int mmc_tsl_pod_map(mmc_ctx* ctx, int tsl, uint64_t n, const float* x, float* y)
 int mmc_tsl_sample_surrogate(mmc_ctx* ctx, int tsl, uint64_t n,
                              const double* E, const double* T, uint32_t* rng,
                              double* alpha, double* beta);
+/* mmc_tsl_sample_surrogate without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The network's inputs, outputs and error word live in a static device
+ * array of 6 MiB per GPU (held until the process ends), so n is at most
+ * 262144 rows; a larger n returns MMC_ERR_ARG before anything runs.  The
+ * results equal mmc_tsl_sample_surrogate's bit for bit.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other host
+ * threads, other contexts) race. */
+int mmc_tsl_sample_surrogate_noalloc(mmc_ctx* ctx, int tsl, uint64_t n,
+                                     const double* E, const double* T, uint32_t* rng,
+                                     double* alpha, double* beta);
 /* Training on device samples: x [n x 4] normalised inputs, y [n x 2]
  * normalised targets (device pointers).  Runs `steps` Adam steps over
  * minibatches of `batch` rows drawn in order (wrapping), returns the mean

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/minimc_amd.h"
@@ -1256,3 +1257,73 @@ int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap) {
   m.stage_cap = cap;
   return MMC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Surrogate sampling without allocation (mmc_tsl_sample_surrogate_noalloc),
+// for callers that cannot allocate device memory.  The staging of
+// mmc_tsl_sample_surrogate (the network's inputs and outputs, the error
+// word) lives in `scratch`, a file-scope device array that each GPU holds
+// until the process ends: kNoAllocRows rows, 6 MiB.  Its host-side name is a
+// stand-in, not a device address; MlpScratchInit looks the address up once
+// per device (hipGetSymbolAddress, with that device current) and every
+// launch below uses that address.  Every call on one GPU shares the array:
+// calls that overlap on one GPU race, and each call clears the error word
+// before its kernels run.
+constexpr uint64_t kNoAllocRows = 1u << 18;
+constexpr size_t kScratchX = 0;                                                // float4 rows
+constexpr size_t kScratchY = kScratchX + sizeof(float) * 4 * kNoAllocRows;    // float2 rows
+constexpr size_t kScratchErr = kScratchY + sizeof(float) * 2 * kNoAllocRows;  // u32
+constexpr size_t kScratchBytes = kScratchErr + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static void* g_scratch[kScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The address of `scratch` on `device`, which must be current; 0 on success.
+int MlpScratchInit(int device) {
+  if (device < 0 || device >= kScratchDevices) return MMC_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return MMC_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MMC_ERR_HIP;
+  g_scratch[device] = p;
+  return MMC_OK;
+}
+
+extern "C" int mmc_tsl_sample_surrogate_noalloc(mmc_ctx* ctx, int tsl, uint64_t n,
+                                                const double* E, const double* T,
+                                                uint32_t* rng, double* alpha, double* beta) {
+  if (!ctx || !E || !T || !rng || !alpha || !beta) return MMC_ERR_ARG;
+  if (tsl < 0 || tsl >= MmcTslCount(ctx)) return MmcFail(ctx, MMC_ERR_ARG, "no such tsl table");
+  MlpState& m = MmcMlp(ctx);
+  if (!m.ready) return MmcFail(ctx, MMC_ERR_UNSUPPORTED, "no surrogate network set");
+  if (n > kNoAllocRows)
+    return MmcFail(ctx, MMC_ERR_ARG,
+                   "batch larger than the static scratch of the no-allocation sampler "
+                   "(262144 rows)");
+  if (n == 0) return MMC_OK;
+  const int dev = MmcDevice(ctx);
+  if (hipSetDevice(dev) != hipSuccess) return MmcFail(ctx, MMC_ERR_HIP, "hipSetDevice failed");
+  if (dev >= kScratchDevices || (!g_scratch[dev] && MlpScratchInit(dev)))
+    return MmcFail(ctx, MMC_ERR_HIP, "no address for the static scratch");
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[dev]);
+  float* xin = reinterpret_cast<float*>(base + kScratchX);
+  float* yout = reinterpret_cast<float*>(base + kScratchY);
+  uint32_t* err = reinterpret_cast<uint32_t*>(base + kScratchErr);
+  hipStream_t s = MmcStream(ctx);
+  hipLaunchKernelGGL(KClearWord, dim3(1), dim3(1), 0, s, err);
+  const uint32_t nb = uint32_t((n + 255) / 256);
+  hipLaunchKernelGGL(KSurrogateBatch, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
+                     uint32_t(n), E, T, rng, xin);
+  LaunchInfer(m, xin, yout, uint32_t(n), s);
+  hipLaunchKernelGGL(KSurrogateFinish, dim3(nb), dim3(256), 0, s, MmcWorld(ctx), m, tsl,
+                     uint32_t(n), E, T, rng, yout, alpha, beta, err);
+  uint32_t herr = 0;
+  if (hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return MmcFail(ctx, MMC_ERR_HIP, "error word readback failed");
+  if (const int rc = SyncChecked(ctx, "mmc_tsl_sample_surrogate_noalloc")) return rc;
+  if (herr) return MmcFail(ctx, MMC_ERR_RESAMPLE, "POD fallback failed");
+  return MMC_OK;
+}
+

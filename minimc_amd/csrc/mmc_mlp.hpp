@@ -40,6 +40,9 @@ void MlpFree(MlpState& m);
 struct mmc_ctx;
 // Staging buffers of the surrogate step for `cap` particle slots.
 int MlpEnsureStaging(mmc_ctx* ctx, uint32_t cap);
+// The device address of the no-allocation sampler's static scratch on
+// `device` (current), looked up once per device; 0 on success.
+int MlpScratchInit(int device);
 namespace mmc {
 // One thermal-scattering step of the tracker with the surrogate: one network
 // kernel that gathers (log E, T, u1, u2) of the TSL queue and applies the

@@ -894,6 +894,11 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     *out = ctx;  // keep the message available through mmc_last_error
     return rc;
   }
+  // the no-allocation sampler's static scratch: its address on this device
+  if (MlpScratchInit(ctx->device)) {
+    *out = ctx;
+    return Fail(ctx, MMC_ERR_HIP, "no address for the static scratch");
+  }
   // The CE kernels stage the energy hash (up to 32 KiB) into LDS once per
   // block: cap their grid at the blocks that are resident at once, so that
   // each block amortises the staging over several grid-stride passes.
