@@ -147,21 +147,30 @@ def _profiled_traffic(kernel, workload):
     return None
 
 
+def _tail_pod(ev):
+    """Thermal collisions the tail kernel sampled with POD itself: every
+    thermal collision is either a network row, a network fallback (POD
+    outside the tail kernel) or one of these."""
+    return ev["tsl_collisions"] - ev["mlp_rows"] - ev["tsl_fallbacks"]
+
+
 def _tail_roofline(cfg, kt, ev):
     """The history-based tail kernel against the HBM roofline: the
     algorithmic bytes of the events it ran (the event kernels' per-event
-    bytes) over its HIP-event time."""
+    bytes; a thermal collision it parks for the network costs the tail
+    kernel only its collision) over its HIP-event time."""
     if "tail" not in kt or kt["tail"][1] == 0:
         return None
     ms, launches = kt["tail"]
     b = BYTES[cfg]
     nbytes = (ev["tail_advances"] * b["advance"] + ev["tail_crossings"] * b["cross"] +
-              ev["tail_collisions"] * b["collide"] + ev["tail_tsl"] * b["tsl"])
+              ev["tail_collisions"] * b["collide"] + _tail_pod(ev) * b["tsl"])
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "tail", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "events": {k: ev["tail_" + k] for k in ("advances", "crossings", "collisions",
-                                                    "tsl")},
+            "events": dict({k: ev["tail_" + k] for k in ("advances", "crossings",
+                                                         "collisions", "tsl")},
+                           tsl_pod=_tail_pod(ev)),
             "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
             "avg_launch_ms": ms / max(launches, 1)}
 
@@ -367,7 +376,14 @@ def run_c5(args):
     tp.close()
     # surrogate tracker: the shipped network (minimc_amd/data), or one trained
     # here outside every timed region when that is missing
-    ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n, **opts)
+    t0 = time.perf_counter()
+    if args.tail_network is not None:
+        sopts, stable = dict(opts, tail_network=args.tail_network), []
+    else:
+        sopts, stable = autotune.tune_tail_network(
+            pr, opts, n=n, cycles=3, prepare=lambda t: surrogate.cached_network(t))
+    t_tune += time.perf_counter() - t0
+    ts = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=n, **sopts)
     t0 = time.perf_counter()
     net_path = surrogate.SHIPPED
     _, trained_here = surrogate.cached_network(ts, path=net_path, epochs=args.train_epochs,
@@ -380,8 +396,9 @@ def run_c5(args):
     kt = sur.pop("_kt")
     pod_kt = pod.pop("_kt")
     # the surrogate run's event-kernel rows: the network sampled mlp_rows
-    # thermal collisions; the POD sampler the parked fallbacks (in the
-    # thermal-scattering kind) and every thermal collision of the tail kernel
+    # thermal collisions (of the event iterations and of the tail's rounds);
+    # the POD sampler the network's fallbacks (in the thermal-scattering
+    # kind) and the thermal collisions of the tail kernel's last launch
     counts = {"advance": ev["advances"] - ev["tail_advances"],
               "collide": ev["collisions"] - ev["tail_collisions"],
               "cross": ev["crossings"] - ev["tail_crossings"],
@@ -410,13 +427,14 @@ def run_c5(args):
                     "batches of both runs" % FOM_BATCH,
         "thermal_sampling": {"network_rows": ev["mlp_rows"],
                              "pod_fallbacks": ev["tsl_fallbacks"],
-                             "pod_in_tail_kernel": ev["tail_tsl"],
+                             "pod_in_tail_kernel": _tail_pod(ev),
                              "network_fraction": ev["mlp_rows"] / max(ev["tsl_collisions"], 1)},
         "histories_per_s_ratio": sur["histories_per_s"] / pod["histories_per_s"],
         "k_diff_sigma": float(dk),
         "surrogate": sur, "pod": pod, "tsl_batch": batch,
-        "autotune": {"best": opts, "seconds": t_tune,
-                     "ms_per_cycle": [[o, s * 1e3] for o, s in table]},
+        "autotune": {"best": opts, "best_surrogate": sopts, "seconds": t_tune,
+                     "ms_per_cycle": [[o, s * 1e3] for o, s in table],
+                     "surrogate_ms_per_cycle": [[o, s * 1e3] for o, s in stable]},
         "network": {"trained_here": trained_here, "seconds": t_net,
                     "path": os.path.relpath(net_path, ROOT)},
         "roofline": _roofline("c5", kt, counts),
@@ -437,6 +455,8 @@ def main():
     ap.add_argument("--tail-threshold", type=int, default=None,
                     help="with --graph-iters: skip the autotuner")
     ap.add_argument("--graph-iters", type=int, default=None)
+    ap.add_argument("--tail-network", type=int, default=None,
+                    help="surrogate tracker's tail_network option (default: tuned)")
     ap.add_argument("--compact-interval", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:

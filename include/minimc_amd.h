@@ -301,7 +301,11 @@ typedef struct mmc_options {
                            (0 = 256)                                      */
   int32_t tsl_block;    /* threads per block of the POD thermal-scattering
                            kernel: 256 or 384 (0 = 256)                   */
-  int32_t pad_o;
+  int32_t tail_network; /* surrogate mode, k > 0: while at least k
+                           particles are live, the tail runs in rounds that
+                           park thermal collisions for the network; below
+                           k the tail kernel samples them with POD.  0 (the
+                           default): the tail samples with POD throughout */
 } mmc_options;
 /* Reference behaviours that are reproduced only on request.
  * MMC_QUIRK_NUDGE_CROSSING: after a surface crossing the new cell is the

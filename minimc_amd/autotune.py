@@ -9,6 +9,10 @@ problem (history length distribution, table sizes) and on the device:
                   live-queue length
   compact_interval  bank compaction policy (0: when half empty, k: every k
                   iterations, -1: never)
+  graphs          captured batches (k-eigenvalue)
+and, for the surrogate tracker (tune_tail_network),
+  tail_network    live count down to which the tail parks thermal
+                  collisions for the network (0: POD in the tail)
 
 `tune()` times short runs of the actual problem for each candidate
 (coordinate descent: the tail threshold, the readback interval, then the
@@ -29,6 +33,9 @@ from . import abi
 TAIL_CANDIDATES = (-1, 4096, 16384, 32768, 65536, 131072, 262144)
 ITER_CANDIDATES = (4, 8, 16, 32)
 COMPACT_CANDIDATES = (-1, 1, 2, 4)  # 0 (compact when half empty) is the start
+# surrogate mode: live count down to which the tail runs in rounds that park
+# thermal collisions for the network (0: the tail samples them with POD)
+TAIL_NETWORK_CANDIDATES = (0, 16384, 4096)
 # A candidate replaces the best so far only if it is this much faster: run to
 # run timing noise is about 1 %, and near-ties keep the earlier candidate.
 MARGIN = 0.015
@@ -134,6 +141,38 @@ def tune(problem, n=None, cycles=3, tsl_mode=abi.MMC_TSL_POD, prepare=None,
                 opts["tail_threshold"], opts["graph_iters"],
                 opts.get("compact_interval", 0), s * 1e3))
         if s < best_s * (1 - MARGIN):
+            best, best_s = dict(opts), s
+    if use_cache:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        cache[key] = {"best": best, "table": table}
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1)
+    return best, table
+
+
+def tune_tail_network(problem, base, n=None, cycles=3, prepare=None, use_cache=True,
+                      log=None):
+    """For the surrogate tracker with launch options `base` (from tune()):
+    the tail_network value, from TAIL_NETWORK_CANDIDATES, that runs fastest
+    (prepare(tracker) must set the network).  Returns (options, table)."""
+    n = n or problem.histories
+    key = "tail_network:" + _signature(problem, abi.MMC_TSL_SURROGATE, n) + \
+        json.dumps(base, sort_keys=True)
+    cache = {}
+    if use_cache and os.path.exists(CACHE):
+        with open(CACHE) as f:
+            cache = json.load(f)
+        if key in cache:
+            return cache[key]["best"], cache[key]["table"]
+    table = []
+    best, best_s = None, float("inf")
+    for tn in TAIL_NETWORK_CANDIDATES:
+        opts = dict(base, tail_network=tn)
+        s = _time_run(problem, opts, n, cycles, abi.MMC_TSL_SURROGATE, prepare)
+        table.append((dict(opts), s))
+        if log:
+            log("tail_network %6d: %.2f ms" % (tn, s * 1e3))
+        if best is None or s < best_s * (1 - MARGIN):
             best, best_s = dict(opts), s
     if use_cache:
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)

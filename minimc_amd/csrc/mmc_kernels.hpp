@@ -1089,33 +1089,41 @@ __global__ void __launch_bounds__(256) KCompact(RunState R, DevBank dst, int ite
 // that most of its lanes wait for, so a step costs one event's latency
 // instead of the sum over the types present in the wave.  A lane whose
 // history ends takes the next queued particle.  Thermal scatterings use the
-// POD sampler here.  Used only where a collision cannot start a new particle
-// (k-eigenvalue, or no fission).
+// POD sampler here, unless kPark.  Used only where a collision cannot start a
+// new particle (k-eigenvalue, or no fission).
 // kThreads: threads per block (the launch option tail_block).  The tail
 // stages more LDS than two blocks of a CU can hold, so its occupancy is one
 // block per CU: 512 threads make that two waves per SIMD (and cap the
 // registers at 256 per lane, arch and acc together).
-template <bool kLds, int kThreads = 256>
+// kPark: one round of the tail with the network sampler.  A thermal
+// collision parks its particle in the thermal-scattering queue of iteration
+// `iter` (QT, ring slot kT) and the lane takes the next particle; the
+// network kernel then scatters the parked particles into the live queue of
+// iteration `iter`, which the next round runs.  Like KAdvance, the round
+// clears the ring row of iteration iter + 1 for the next round.
+template <bool kLds, int kThreads = 256, bool kPark = false>
 __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int iter, int qin) {
   enum { kAdv = 0, kCol = 1, kTsl = 2, kDone = 3 };
   extern __shared__ int lds_hash[];
+  if (kPark && blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
   uint32_t n1;
   const uint32_t n = LiveLength(R, iter - 1, n1);
   if (blockIdx.x * blockDim.x >= n) return;
   // kLds: the hash, the totals, the sampler's tables and the World's
-  // descriptors in LDS
+  // descriptors in LDS; a parking round does not sample, and its LDS holds
+  // the hash, the totals and the descriptors only (the event kernels' LDS)
   const int* hash = kLds ? StageHash(Wg, lds_hash) : nullptr;
   double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
   const double* Ht = kLds ? StageHot(Wg.tot, lds_tot, Wg.tot_n) : Wg.tot;
-  const double* H = kLds ? StageHot(Wg.hot, lds_tot + Wg.tot_n, Wg.hot_n) : Wg.hot;
+  const double* H = kLds && !kPark ? StageHot(Wg.hot, lds_tot + Wg.tot_n, Wg.hot_n) : Wg.hot;
   const DevWorld W = kLds ? DescInLds(Wg, reinterpret_cast<unsigned char*>(
-                                              lds_tot + Wg.tot_n + Wg.hot_n))
+                                              lds_tot + Wg.tot_n + (kPark ? 0 : Wg.hot_n)))
                           : Wg;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   int st = q < n ? kAdv : kDone;
   uint32_t p = st == kAdv ? LiveAt(R, qin, n1, q) : 0u;
-  unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0;
+  unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0, c_park = 0;
   unsigned long long c_tail = st == kAdv ? 1 : 0;
   for (;;) {
     int pick = -1, most = 0;
@@ -1139,10 +1147,28 @@ __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int i
       c_coll++;
       bool alive, to_t;
       CollideEvent(W, R, p, hash, Ht, nullptr, nullptr, alive, to_t);
-      st = to_t ? kTsl : (alive ? kAdv : kDone);
-    } else {
+      if (kPark) {
+        // one queue push for the wave's parked lanes (the lanes active in
+        // this branch; the first of them adds the count)
+        const uint64_t m = __ballot(to_t);
+        if (m) {
+          const int lead = __ffsll(static_cast<unsigned long long>(m)) - 1;
+          uint32_t base = 0;
+          if (__lane_id() == uint32_t(lead))
+            base = atomicAdd(Ring(R, iter, kT), uint32_t(__popcll(m)));
+          base = __shfl(base, lead);
+          if (to_t) R.QT[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = p;
+        }
+        c_park += to_t ? 1 : 0;
+        st = alive && !to_t ? kAdv : kDone;
+      } else {
+        st = to_t ? kTsl : (alive ? kAdv : kDone);
+      }
+    } else if constexpr (!kPark) {
       c_tsl++;
       st = TslPodEvent(W, R, p, H) ? kAdv : kDone;
+    } else {
+      st = kDone;  // (a parking round has no thermal step)
     }
     if (st == kDone) {
       q += stride;
@@ -1162,7 +1188,7 @@ __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int i
     atomicAdd(R.ev + kEvTail, c_tail);
     atomicAdd(R.ev + kEvTailAdv, c_adv);
     atomicAdd(R.ev + kEvTailCol, c_coll);
-    atomicAdd(R.ev + kEvTailTsl, c_tsl);
+    atomicAdd(R.ev + kEvTailTsl, c_tsl + c_park);
     atomicAdd(R.ev + kEvTailCross, c_cross);
   }
 }

@@ -56,6 +56,7 @@ struct mmc_ctx {
   int n_hash0 = 0;             // LDS entries staged by the CE kernels
   uint32_t lds_grid = 1024;    // grid cap of the LDS-staging kernels
   uint32_t tail_grid = 512;    // blocks of the tail kernel resident at once
+  uint32_t tail_park_grid = 512;  // and of its parking rounds
   int tail_block = 256;        // threads per block of KTail (256 or 512)
   int tsl_block = 256;         // threads per block of KTslPod (256 or 384)
   size_t hot_lds = 0;          // LDS bytes of KTslPod<true> (0: it does not stage)
@@ -63,6 +64,7 @@ struct mmc_ctx {
   size_t ev_lds = 0;           // LDS bytes of KAdvance / KCollide
   bool ev_tot_lds = false;     // ... which stage the totals and descriptors after the hash
   size_t tail_lds = 0;         // LDS bytes of KTail<true> (0: KTail<false>, no LDS)
+  size_t tail_park_lds = 0;    // and of its parking rounds (hash, totals, descriptors)
   DevBank alt{};               // second bank: target of the compaction
   // bank
   uint32_t cap = 0;
@@ -748,6 +750,12 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
     const size_t need = hash_lds + tot_lds + hot + desc_lds;
     ctx->tail_lds =
         desc_lds && hot <= kHotLdsMax && need + tail_static <= kLdsPerBlock ? need : 0;
+    const size_t park_static =
+        std::max(StaticLds(reinterpret_cast<const void*>(KTail<true, 256, true>)),
+                 StaticLds(reinterpret_cast<const void*>(KTail<true, 512, true>)));
+    const size_t park = hash_lds + tot_lds + desc_lds;
+    ctx->tail_park_lds = desc_lds && tot_lds <= kTotLdsMax &&
+                                 park + park_static <= kLdsPerBlock ? park : 0;
   }
   ctx->S = DevAlloc<double>(ctx, nb);
   ctx->S2 = DevAlloc<double>(ctx, nb);
@@ -883,6 +891,10 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     return MMC_ERR_ARG;
   }
   ctx->tail_block = ctx->opt.tail_block ? ctx->opt.tail_block : 256;
+  if (ctx->opt.tail_network < 0) {
+    *out = ctx;
+    return Fail(ctx, MMC_ERR_ARG, "tail_network must be 0 or a particle count");
+  }
   ctx->tsl_block = ctx->opt.tsl_block ? ctx->opt.tsl_block : 256;
   if (hipSetDevice(ctx->device) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -939,6 +951,19 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
                                          &occ_t, KTail<false>, tb, 0));
     if (et != hipSuccess) occ_t = 1;
     ctx->tail_grid = uint32_t(std::max(1, occ_t)) * uint32_t(n_cu);
+    int occ_p = 0;
+    const size_t pl = ctx->tail_park_lds;
+    const hipError_t ep =
+        tb == 512 ? (pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &occ_p, KTail<true, 512, true>, tb, pl)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &occ_p, KTail<false, 512, true>, tb, 0))
+                  : (pl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &occ_p, KTail<true, 256, true>, tb, pl)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &occ_p, KTail<false, 256, true>, tb, 0));
+    if (ep != hipSuccess) occ_p = 1;
+    ctx->tail_park_grid = uint32_t(std::max(1, occ_p)) * uint32_t(n_cu);
     // KTslPod with the hot block in LDS: resident blocks only
     int occ_s = 0;
     if (ctx->hot_lds &&
@@ -1306,6 +1331,74 @@ static int GraphBatch(mmc_ctx* ctx, int& iter, int& qin, uint32_t grid, int K,
   return MMC_OK;
 }
 
+// One launch of the tail kernel over the live queue of iteration iter - 1 (at
+// most `bound` particles); park: a round that parks thermal collisions for
+// the network instead of sampling them.
+template <bool kLds, int kThreads, bool kPark>
+static void TailLaunch(mmc_ctx* ctx, dim3 grid, int iter, int qin) {
+  const size_t lds = !kLds ? 0 : kPark ? ctx->tail_park_lds : ctx->tail_lds;
+  hipLaunchKernelGGL((KTail<kLds, kThreads, kPark>), grid, dim3(kThreads), lds, ctx->stream,
+                     ctx->W, ctx->R, iter, qin);
+}
+template <int kThreads, bool kPark>
+static void TailLaunchT(mmc_ctx* ctx, dim3 grid, int iter, int qin) {
+  if (kPark ? ctx->tail_park_lds : ctx->tail_lds)
+    TailLaunch<true, kThreads, kPark>(ctx, grid, iter, qin);
+  else
+    TailLaunch<false, kThreads, kPark>(ctx, grid, iter, qin);
+}
+static void LaunchTail(mmc_ctx* ctx, int iter, int qin, uint32_t bound, bool park) {
+  const uint32_t tb = uint32_t(ctx->tail_block);
+  const uint32_t resident = park ? ctx->tail_park_grid : ctx->tail_grid;
+  const dim3 grid(std::max(1u, std::min(uint32_t((bound + tb - 1) / tb), resident)));
+  ProfBegin(ctx, MMC_K_TAIL);
+  if (tb == 512)
+    park ? TailLaunchT<512, true>(ctx, grid, iter, qin)
+         : TailLaunchT<512, false>(ctx, grid, iter, qin);
+  else
+    park ? TailLaunchT<256, true>(ctx, grid, iter, qin)
+         : TailLaunchT<256, false>(ctx, grid, iter, qin);
+  ProfEnd(ctx);
+}
+
+// Tail rounds with the network (surrogate mode): each round runs the live
+// particles to their next thermal collision (parked) or to the end of their
+// history, then the network kernel scatters the parked ones into the next
+// round's live queue.  Rounds run in groups of K, each group ending with the
+// POD sampler for the network's fallbacks and a read of the live count,
+// until fewer than `until` particles are live; `bound` is then the live
+// count.
+static int TailRounds(mmc_ctx* ctx, int& iter, int& qin, uint32_t& bound, uint32_t until,
+                      int K) {
+  uint32_t* const ring = ctx->host_rb;
+  uint32_t* const misc = ctx->host_rb + kRing * kRingStride;
+  while (bound >= until && bound > 0) {
+    for (int k = 0; k < K; k++) {
+      LaunchTail(ctx, iter, qin, bound, true);
+      if (MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, 0, ctx->stream, ProfMark, ctx))
+        return Fail(ctx, MMC_ERR_HIP, "surrogate step failed");
+      iter++;
+      qin ^= 1;
+    }
+    ProfBegin(ctx, MMC_K_TSL);
+    const int rf =
+        MlpFlushFallback(ctx->mlp, ctx->W, ctx->R, iter, qin, ctx->hot_lds, ctx->stream);
+    ProfEnd(ctx);
+    if (rf) return Fail(ctx, MMC_ERR_HIP, "surrogate fallback failed");
+    if (const int lc = LaunchCheck(ctx, "tail rounds")) return lc;
+    HIPCHK(hipMemcpyAsync(ring, ctx->R.ring, sizeof(uint32_t) * (kRing * kRingStride + 16),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ProfCollect(ctx);
+    ctx->cnt.iterations += K;
+    const int rc = CheckMisc(ctx, misc);
+    if (rc) return rc;
+    const uint32_t* row = ring + ((iter - 1 + kRing) % kRing) * kRingStride;
+    bound = row[kQ] + row[kQ2];
+  }
+  return MMC_OK;
+}
+
 // Runs event iterations until no particle is alive.  The live-queue length
 // is read back every `K` iterations; in between, kernels read queue lengths
 // from device memory and grid-stride over them.
@@ -1339,27 +1432,21 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
   // cycle to cycle), with compaction every iteration, at readbacks or never
   // (KCompact every k > 1 iterations would tie a batch to iter mod k).
   const bool graphs = ctx->opt.graphs > 0 && ctx->R.keig && every <= 1;
+  // the tail's thermal collisions go to the network while at least
+  // net_tail_at particles are live (an option: every round waits for its
+  // slowest particle, and the rounds cost more than the network saves on
+  // the benchmark problems)
+  const bool net_tail = surrogate && ctx->opt.tail_network > 0;
+  const uint32_t net_tail_at = uint32_t(std::max(ctx->opt.tail_network, 1));
   uint32_t in_use = n0;
   for (;;) {
     if (tail_ok && bound < tail_at && iter > 0) {
-      ProfBegin(ctx, MMC_K_TAIL);
-      const uint32_t tb = uint32_t(ctx->tail_block);
-      const dim3 tgrid(std::min(uint32_t((bound + tb - 1) / tb), ctx->tail_grid));
-      if (tb == 512) {
-        if (ctx->tail_lds)
-          hipLaunchKernelGGL((KTail<true, 512>), tgrid, dim3(tb), ctx->tail_lds, ctx->stream,
-                             ctx->W, ctx->R, iter, qin);
-        else
-          hipLaunchKernelGGL((KTail<false, 512>), tgrid, dim3(tb), 0, ctx->stream, ctx->W,
-                             ctx->R, iter, qin);
-      } else if (ctx->tail_lds) {
-        hipLaunchKernelGGL(KTail<true>, tgrid, dim3(kBlock), ctx->tail_lds, ctx->stream,
-                           ctx->W, ctx->R, iter, qin);
-      } else {
-        hipLaunchKernelGGL(KTail<false>, tgrid, dim3(kBlock), 0, ctx->stream, ctx->W, ctx->R,
-                           iter, qin);
+      if (net_tail) {
+        const int rc = TailRounds(ctx, iter, qin, bound, net_tail_at, K);
+        if (rc) return rc;
+        if (bound == 0) break;
       }
-      ProfEnd(ctx);
+      LaunchTail(ctx, iter, qin, bound, false);
       if (const int lc = LaunchCheck(ctx, "tail kernel")) return lc;
       HIPCHK(hipMemcpyAsync(misc, ctx->R.misc, kMiscBytes, hipMemcpyDeviceToHost,
                             ctx->stream));

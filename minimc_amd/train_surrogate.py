@@ -1,7 +1,9 @@
 """Trains the thermal-scattering surrogate on the GPU and assesses it.
 
 python -m minimc_amd.train_surrogate [--samples 26] [--epochs 3] [--batch 8192]
-                                     [--lr 2e-3] [--out PATH] [--eval-only]
+                                     [--lr 2e-3] [--rounds 1] [--gamma 1] [--lr-floor 1e-5]
+                                     [--weighting local|bins] [--assess-n 2000000]
+                                     [--out PATH] [--eval-only]
 
 Training data: 2^samples POD transport-map targets generated on the device
 from the synthetic S(alpha, beta, T) tables.  The assessment compares the
@@ -74,6 +76,11 @@ def main():
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--lr-floor", type=float, default=1e-5)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--weighting", choices=("local", "bins"), default="local")
+    ap.add_argument("--e-uniform", type=float, default=surrogate.E_UNIFORM_FRACTION,
+                    help="fraction of training energies drawn uniformly in E")
+    ap.add_argument("--assess-n", type=int, default=2000000,
+                    help="samples per assessment point and sampler")
     args = ap.parse_args()
     pr = configs.tsl_box(configs.tsl_tables(), T=600.0)
     t = Tracker(pr, max_in_flight=1 << 16)
@@ -86,16 +93,18 @@ def main():
         t0 = time.perf_counter()
         losses = s.train(n_samples=1 << args.samples, batch=args.batch, epochs=args.epochs,
                          lr=args.lr, log=lambda m: print(m, flush=True), rounds=args.rounds,
-                         gamma=args.gamma, lr_floor=args.lr_floor, seed=args.seed)
+                         gamma=args.gamma, lr_floor=args.lr_floor, seed=args.seed,
+                         weighting=args.weighting, e_uniform=args.e_uniform)
         dt = time.perf_counter() - t0
         steps = args.epochs * ((1 << args.samples) // args.batch)
         info = dict(train_seconds=dt, steps=steps, samples=1 << args.samples,
                     epochs=args.epochs, batch=args.batch, lr=args.lr, rounds=args.rounds,
                     gamma=args.gamma, lr_floor=args.lr_floor, seed=args.seed,
+                    weighting=args.weighting, e_uniform=args.e_uniform,
                     final_loss=losses[-1],
                     train_tflops=steps * args.batch * 3 * 2 * surrogate.n_params() / dt / 1e12)
         s.save(out, info=info)
-    rows = assess(t)
+    rows = assess(t, n=args.assess_n)
     for r in rows:
         print("E=%6.3f eV T=%6.1f  W1 beta %.4f alpha %.4f  KS beta %.4f alpha %.4f" % (
             r["E_eV"], r["T"], r["w1_beta"], r["w1_alpha"], r["ks_beta"], r["ks_alpha"]),

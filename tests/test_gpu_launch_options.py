@@ -100,6 +100,54 @@ def test_surrogate_schedules_give_identical_cycles():
         assert c["tsl_fallbacks"] == rc["tsl_fallbacks"]
 
 
+def test_surrogate_tail_rounds_match_event_iterations():
+    """With the network serving every thermal collision of the tail
+    (tail_network=1: tail rounds until no particle is left), the tail gives
+    the same cycles as event iterations alone: the tail kernel runs the same
+    event functions, and the network kernel the same rows.  Rounds down to
+    4096 live particles (then POD) and the default (POD throughout the tail)
+    are checked for their counters."""
+    require_gpu()
+    from minimc_amd import surrogate
+    from minimc_amd.tracker import Tracker
+    pr = configs.c5_multizone(histories=20000)
+    runs = {}
+    for name, opts in (("events", dict(tail_threshold=-1)),
+                       ("rounds", dict(tail_threshold=8192, tail_network=1)),
+                       ("rounds_early", dict(tail_threshold=1 << 30, tail_network=1,
+                                             graph_iters=4)),
+                       ("pod_tail", dict(tail_threshold=1 << 30)),
+                       ("to_4096", dict(tail_threshold=1 << 30, tail_network=4096))):
+        t = Tracker(pr, tsl_mode=abi.MMC_TSL_SURROGATE, max_in_flight=1 << 16,
+                    compact_interval=1, **opts)
+        surrogate.cached_network(t)  # the shipped network
+        t.set_entropy_mesh(*pr.entropy_mesh)
+        t.keig_init(pr.histories)
+        runs[name] = ([t.run_cycle(active=c >= 1) for c in range(4)], t.counters())
+        t.close()
+
+    def pod_in_tail(c):
+        return c["tsl_collisions"] - c["mlp_rows"] - c["tsl_fallbacks"]
+
+    ref, rc = runs["events"]
+    assert rc["tail_particles"] == 0 and pod_in_tail(rc) == 0
+    for name in ("rounds", "rounds_early"):
+        st, c = runs[name]
+        for a, b in zip(ref, st):
+            for k in KEYS:
+                assert a[k] == b[k], (name, k, a[k], b[k])
+        for k in ("collisions", "advances", "tsl_collisions", "mlp_rows", "tsl_fallbacks"):
+            assert c[k] == rc[k], (name, k)
+        assert c["tail_tsl"] > 0 and pod_in_tail(c) == 0, (name, c)
+    # POD throughout the tail: the tail kernel samples its thermal collisions
+    _, c = runs["pod_tail"]
+    assert pod_in_tail(c) == c["tail_tsl"] > 0, c
+    # rounds down to 4096 live particles, then POD
+    _, c = runs["to_4096"]
+    assert 0 < pod_in_tail(c) < c["tail_tsl"], c
+    assert c["mlp_rows"] > runs["pod_tail"][1]["mlp_rows"], c
+
+
 def _ce_slab_with_sensitivity_and_flux():
     """The continuous-energy slab (reduced grid) with a total cross-section
     perturbation, the sensitivity of both leakage tallies to it, and a flux
