@@ -148,9 +148,13 @@ def _profiled_traffic(kernel, workload):
 
 
 def _tail_pod(ev):
-    """Thermal collisions the tail kernel sampled with POD itself: every
-    thermal collision is either a network row, a network fallback (POD
-    outside the tail kernel) or one of these."""
+    """Thermal collisions the tail kernel sampled with POD itself.  In the
+    POD tracker, all of the tail's; in the surrogate tracker every thermal
+    collision is either a network row, a network fallback (POD outside the
+    tail kernel) or one of these (the tail's parked collisions are network
+    rows)."""
+    if ev["mlp_rows"] == 0:
+        return ev["tail_tsl"]
     return ev["tsl_collisions"] - ev["mlp_rows"] - ev["tsl_fallbacks"]
 
 
@@ -354,6 +358,11 @@ def tsl_batch_rates(tracker, n=1 << 22, reps=5, seed=4):
             "speedup": out["surrogate"] / out["pod"]}
 
 
+def _progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def run_c5(args):
     from minimc_amd import abi, configs, surrogate
     from minimc_amd.tracker import Tracker
@@ -370,10 +379,12 @@ def run_c5(args):
     else:
         opts, table = autotune.tune(pr, n=n, cycles=3)
     t_tune = time.perf_counter() - t0
+    _progress("launch options %s (%.1f s)" % (opts, t_tune))
     # plain POD tracker
     tp = Tracker(pr, tsl_mode=abi.MMC_TSL_POD, max_in_flight=n, **opts)
     pod = _keig_leg(pr, tp, args.warmup, args.steps)
     tp.close()
+    _progress("POD tracker: %.4g histories/s" % pod["histories_per_s"])
     # surrogate tracker: the shipped network (minimc_amd/data), or one trained
     # here outside every timed region when that is missing
     t0 = time.perf_counter()
@@ -389,7 +400,9 @@ def run_c5(args):
     _, trained_here = surrogate.cached_network(ts, path=net_path, epochs=args.train_epochs,
                                                batch=32768, lr=3e-3)
     t_net = time.perf_counter() - t0
+    _progress("surrogate launch options %s" % sopts)
     sur = _keig_leg(pr, ts, args.warmup, args.steps)
+    _progress("surrogate tracker: %.4g histories/s" % sur["histories_per_s"])
     batch = tsl_batch_rates(ts)
     ts.close()
     ev = sur["events"]
@@ -480,6 +493,7 @@ def main():
     out.update(res)
     if not args.no_cpu:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        _progress("CPU baseline on %d threads" % threads)
         out["cpu_baseline"] = cpu_baseline(args.config, threads)
     else:
         out["cpu_baseline"] = None
