@@ -1483,10 +1483,7 @@ static int RunEvents(mmc_ctx* ctx, uint32_t n0) {
       in_use = live;
     }
   }
-  // the next run starts with ring slot -1 = kRing-1
-  if (iter % kRing != 0) {
-    // keep the ring consistent for the next call: it is fully reset there
-  }
+  // (the next run resets the ring: ResetRun)
   return MMC_OK;
 }
 

@@ -60,3 +60,45 @@ def test_loss_weights_equalise_bin_variance():
     prod = tab[:, 0] * var0
     assert prod.max() / prod.min() < 1.1  # equalised (sampling noise only)
     assert tab[:, 1].max() / tab[:, 1].min() < 1.35  # nothing to equalise (noise only)
+
+
+def test_alpha_range_kinematics():
+    """alpha_min, alpha_max = (sqrt E -+ sqrt E')^2 / (A kT), E' = E + beta kT:
+    their product is (beta / A)^2 and their sum 2 (E + E') / (A kT)."""
+    rng = np.random.default_rng(2)
+    E = np.exp(rng.uniform(np.log(1e-11), np.log(4e-6), 1000))
+    T = rng.uniform(300, 1200, 1000)
+    kT = surrogate.K_B * T
+    beta = rng.uniform(-E / kT, 20.0)
+    A = 0.99916733
+    y1 = beta * kT / (E + kT)
+    lo, hi = surrogate.alpha_range(E, T, y1, A)
+    Ep = E + beta * kT
+    assert np.allclose(lo * hi, (beta / A) ** 2, rtol=1e-9, atol=1e-30)
+    assert np.allclose(lo + hi, 2 * (E + Ep) / (A * kT), rtol=1e-9)
+    assert np.all(lo <= hi)
+
+
+def test_local_spread_interpolation_is_bilinear():
+    """local_spread_at interpolates node tables bilinearly in (ln E, T): a
+    table linear in both node indices is reproduced exactly between nodes,
+    and the nodes themselves are hit."""
+    nE, nT = 6, 5
+    i, j = np.meshgrid(np.arange(nE), np.arange(nT), indexing="ij")
+    t1 = 2.0 + 0.5 * i + 0.25 * j
+    t2 = 1.0 + i * 0.1 + j * 0.3
+    rng = np.random.default_rng(3)
+    x = np.zeros((500, 4), dtype=np.float64)
+    fe = rng.uniform(0, nE - 1, 500)
+    ft = rng.uniform(0, nT - 1, 500)
+    lo, hi = np.log(surrogate.E_LO), np.log(surrogate.E_HI)
+    x[:, 0] = lo + fe / (nE - 1) * (hi - lo)
+    x[:, 1] = surrogate.T_LO + ft / (nT - 1) * (surrogate.T_HI - surrogate.T_LO)
+    s1, s2 = surrogate.local_spread_at((t1, t2), x)
+    assert np.allclose(s1, 2.0 + 0.5 * fe + 0.25 * ft, rtol=1e-9)
+    assert np.allclose(s2, 1.0 + 0.1 * fe + 0.3 * ft, rtol=1e-9)
+    xn = np.zeros((2, 4))
+    xn[:, 0] = [lo, hi]
+    xn[:, 1] = [surrogate.T_LO, surrogate.T_HI]
+    s1, _ = surrogate.local_spread_at((t1, t2), xn)
+    assert np.allclose(s1, [t1[0, 0], t1[-1, -1]])
