@@ -43,7 +43,7 @@ enum mmc_status {
                               (ThermalScattering.cpp:335-337,460-462)       */
   MMC_ERR_OVERFLOW = 5,    /* a queue or bank capacity was exceeded        */
   MMC_ERR_NODEV = 6,       /* no usable gfx950 device                      */
-  MMC_ERR_UNSUPPORTED = 7, /* feature not available in this build          */
+  MMC_ERR_UNSUPPORTED = 7, /* feature not available or not configured      */
   MMC_ERR_STEPS = 8        /* a history exceeded the step limit            */
 };
 

@@ -121,3 +121,49 @@ def test_full_multizone_k_within_two_sigma_of_pod():
     # the network served most thermal collisions; the POD sampler the rest
     assert c["mlp_rows"] > 0.8 * c["tsl_collisions"], c
     assert c["tsl_fallbacks"] < 0.001 * c["tsl_collisions"], c
+
+
+def test_leakage_spectrum_of_the_continuous_temperature_slab():
+    """The change in leakage spectrum that the network makes: the reference's
+    continuous-temperature benchmark (a 4.33 cm bound-hydrogen slab, linear
+    300 -> 600 K, 0.56 eV pencil beam, delta tracking, leakage in 200 energy
+    bins) run with the POD sampler and with the network, 4e6 histories
+    each.  The network's W1 error shows at this precision: the total leakage
+    moves by about -0.16 % (3.5 sigma of the difference of two independent
+    means).  Asserted: the total leakage within 0.5 %, and each of 10 groups
+    of 20 bins within 4 sigma plus 2 % of the group's leakage."""
+    require_gpu()
+    from minimc_amd.tracker import Tracker
+    pr = configs.continuous_temperature_slab()
+    n = 4_000_000
+    res = {}
+    for mode in (abi.MMC_TSL_POD, abi.MMC_TSL_SURROGATE):
+        t = Tracker(pr, tsl_mode=mode, max_in_flight=1 << 20)
+        if mode == abi.MMC_TSL_SURROGATE:
+            surrogate.cached_network(t)
+        t.run_fixed_source(n)
+        S, S2 = t.tallies(0)
+        res[mode] = (S[1:201], S2[1:201], t.counters())
+        t.close()
+    (Sp, S2p, cp), (Ss, S2s, cs) = res[abi.MMC_TSL_POD], res[abi.MMC_TSL_SURROGATE]
+    assert cs["mlp_rows"] > 0.9 * cs["tsl_collisions"], cs
+
+    def mean_var(S, S2):
+        m = S / n
+        return m, np.maximum(S2 / n - m * m, 0.0) / n
+
+    # each history leaks at most once, so the sums of a group of bins are
+    # its bins' sums
+    (ma, va), (mb, vb) = mean_var(Sp.sum(), S2p.sum()), mean_var(Ss.sum(), S2s.sum())
+    z = (mb - ma) / np.sqrt(va + vb)
+    rel = (mb - ma) / ma
+    print("leakage: POD %.5f surrogate %.5f (%+.3f %%, %.2f sigma)" % (ma, mb, 100 * rel, z))
+    gp, g2p = Sp.reshape(10, 20).sum(1), S2p.reshape(10, 20).sum(1)
+    gs, g2s = Ss.reshape(10, 20).sum(1), S2s.reshape(10, 20).sum(1)
+    (ga, gva), (gb, gvb) = mean_var(gp, g2p), mean_var(gs, g2s)
+    sig = np.sqrt(gva + gvb)
+    print("groups POD:", np.round(ga, 5))
+    print("group change %:", np.round(100 * (gb - ga) / np.maximum(ga, 1e-300), 2))
+    print("group z:", np.round((gb - ga) / np.maximum(sig, 1e-300), 2))
+    assert abs(rel) <= 0.005, rel
+    assert np.all(np.abs(gb - ga) <= 4 * sig + 0.02 * ga), (ga, gb, sig)
