@@ -10,13 +10,15 @@ problem (history length distribution, table sizes) and on the device:
   compact_interval  bank compaction policy (0: when half empty, k: every k
                   iterations, -1: never)
   graphs          captured batches (k-eigenvalue)
+  tail_block, tsl_block  threads per block of the tail kernel and of the POD
+                  thermal-scattering kernel
 and, for the surrogate tracker (tune_tail_network),
   tail_network    live count down to which the tail parks thermal
                   collisions for the network (0: POD in the tail)
 
 `tune()` times short runs of the actual problem for each candidate
-(coordinate descent: the tail threshold, the readback interval, then the
-compaction policy),
+(coordinate descent: the tail threshold, the readback interval, the
+compaction policy, captured batches, then the block sizes),
 and returns the fastest option set (a later candidate must beat the best so
 far by MARGIN).  Results are cached per problem in
 minimc_amd/_build/autotune.json (that folder is built per machine).
@@ -33,6 +35,10 @@ from . import abi
 TAIL_CANDIDATES = (-1, 4096, 16384, 32768, 65536, 131072, 262144)
 ITER_CANDIDATES = (4, 8, 16, 32)
 COMPACT_CANDIDATES = (-1, 1, 2, 4)  # 0 (compact when half empty) is the start
+# threads per block beside the default 256: the tail kernel (two waves per
+# SIMD) and the POD thermal-scattering kernel (three waves per SIMD)
+TAIL_BLOCK_CANDIDATES = (512,)
+TSL_BLOCK_CANDIDATES = (384,)
 # surrogate mode: live count down to which the tail runs in rounds that park
 # thermal collisions for the network (0: the tail samples them with POD)
 TAIL_NETWORK_CANDIDATES = (0, 16384, 4096)
@@ -140,6 +146,21 @@ def tune(problem, n=None, cycles=3, tsl_mode=abi.MMC_TSL_POD, prepare=None,
             log("tail %6d iters %2d compact %2d graphs: %.2f ms" % (
                 opts["tail_threshold"], opts["graph_iters"],
                 opts.get("compact_interval", 0), s * 1e3))
+        if s < best_s * (1 - MARGIN):
+            best, best_s = dict(opts), s
+    # block sizes: the tail kernel (when the tail runs) and the POD
+    # thermal-scattering kernel (when the problem has a thermal-scattering law)
+    blocks = []
+    if best.get("tail_threshold", 0) >= 0:
+        blocks += [("tail_block", b) for b in TAIL_BLOCK_CANDIDATES]
+    if problem.tsl and tsl_mode == abi.MMC_TSL_POD:
+        blocks += [("tsl_block", b) for b in TSL_BLOCK_CANDIDATES]
+    for name, b in blocks:
+        opts = dict(best, **{name: b})
+        s = _time_run(problem, opts, n, cycles, tsl_mode, prepare)
+        table.append((dict(opts), s))
+        if log:
+            log("%s %d: %.2f ms" % (name, b, s * 1e3))
         if s < best_s * (1 - MARGIN):
             best, best_s = dict(opts), s
     if use_cache:
