@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libmmc_h5.so")
+# MMC_H5_LIB: another build of the same library (the sanitizer test loads one)
+LIB_PATH = os.environ.get("MMC_H5_LIB") or os.path.join(_HERE, "_build", "libmmc_h5.so")
 MAX_LEVELS = 8
 _lib = None
 
