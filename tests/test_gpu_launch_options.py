@@ -148,6 +148,28 @@ def test_surrogate_tail_rounds_match_event_iterations():
     assert c["mlp_rows"] > runs["pod_tail"][1]["mlp_rows"], c
 
 
+def test_record_overflow_rerun_gives_identical_cycles():
+    """A k-eigenvalue cycle whose score records overflow their buffer (4 per
+    bank slot to start with) grows the buffer and runs again from the same
+    source.  With a bank of exactly one cycle's histories, the C1 slab's flux
+    mesh (a score per collision) overflows in the first active cycle; a bank
+    16x larger does not.  Cycles and tallies must be bit-identical."""
+    require_gpu()
+    pr = configs.c1_slab(histories=8192)
+    small, c_small = _cycles(pr, dict(), 5, 8192)
+    large, c_large = _cycles(pr, dict(), 5, 8192 * 16)
+    # collisions per active cycle against the small bank's first buffer
+    per_cycle = c_small["collisions"] / 5
+    assert per_cycle > 4 * 8192 + 1024, per_cycle
+    assert per_cycle < 4 * 8192 * 16, per_cycle
+    for a, b in zip(small, large):
+        for k in KEYS:
+            assert a[k] == b[k], (k, a[k], b[k])
+    for k in ("collisions", "crossings", "advances", "fission_sites"):
+        assert c_small[k] == c_large[k], k
+    _same_tallies(c_small["tallies"], c_large["tallies"], "record overflow")
+
+
 def _ce_slab_with_sensitivity_and_flux():
     """The continuous-energy slab (reduced grid) with a total cross-section
     perturbation, the sensitivity of both leakage tallies to it, and a flux
