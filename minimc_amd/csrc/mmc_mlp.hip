@@ -414,9 +414,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
     t_net.add(valid && !fall);
   }
   if (kTrack && io_wave) {
-    t_tsl.flush(io.R.ev + kEvTsl);
-    t_fall.flush(io.R.ev + kEvFallback);
-    t_net.flush(io.R.ev + kEvMlp);
+    t_tsl.flush(Ev(io.R, kEvTsl));
+    t_fall.flush(Ev(io.R, kEvFallback));
+    t_net.flush(Ev(io.R, kEvMlp));
   }
 }
 

@@ -850,7 +850,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
   if (hipHostMalloc(reinterpret_cast<void**>(&ctx->host_rb),
                     sizeof(uint32_t) * (kRing * kRingStride + 16)) != hipSuccess)
     ctx->host_rb = nullptr;
-  R.ev = DevAlloc<unsigned long long>(ctx, kEvN);
+  R.ev = DevAlloc<unsigned long long>(ctx, kEvWords);
   R.capacity = cap;
   if (const int rr = EnsureRecords(ctx, cap * 4 + 1024)) return rr;
   R.fis_cnt = DevAlloc<uint32_t>(ctx, cap);
@@ -864,7 +864,7 @@ static int AllocBank(mmc_ctx* ctx, uint32_t cap) {
       !A.v || !A.w || !A.E || !A.rng || !A.cell || !A.hist || !A.steps ||
       !A.xs_grid || !A.xs_gi || !A.tslT || !A.tslI)
     return Fail(ctx, MMC_ERR_HIP, "out of device memory");
-  HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvN));
+  HIPCHK(hipMemset(R.ev, 0, sizeof(unsigned long long) * kEvWords));
   R.max_steps = uint32_t(ctx->opt.max_steps ? std::min<uint64_t>(ctx->opt.max_steps, 0xffffffffull)
                                             : 1000000ull);
   return MMC_OK;
@@ -1494,9 +1494,14 @@ static int ResetRun(mmc_ctx* ctx) {
 }
 
 static int ReadEvents(mmc_ctx* ctx) {
-  HIPCHK(hipMemcpyAsync(ctx->ev_host, ctx->R.ev, sizeof(ctx->ev_host), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  unsigned long long copies[kEvWords];
+  HIPCHK(hipMemcpyAsync(copies, ctx->R.ev, sizeof(copies), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < kEvN; k++) {
+    unsigned long long s = 0;
+    for (int c = 0; c < kEvSlots; c++) s += copies[c * kEvStride + k];
+    ctx->ev_host[k] = s;
+  }
   ctx->cnt.advances = ctx->ev_host[kEvAdvance];
   ctx->cnt.xs_lookups = ctx->ev_host[kEvAdvance];
   ctx->cnt.collisions = ctx->ev_host[kEvCollide];
@@ -1523,7 +1528,7 @@ extern "C" int mmc_run_fixed_source(mmc_ctx* ctx, const mmc_source_desc* source,
   ctx->R.keig = 0;
   uint32_t chunk_cap = ctx->fixed_fission ? ctx->cap / 2 : ctx->cap;
   uint64_t done = 0;
-  unsigned long long ev_snap[kEvN];
+  unsigned long long ev_snap[kEvWords];
   while (done < n) {
     const uint32_t m = uint32_t(std::min<uint64_t>(n - done, chunk_cap));
     const uint64_t f = first + done;
@@ -1747,7 +1752,7 @@ extern "C" int mmc_run_cycle(mmc_ctx* ctx, int active, mmc_cycle_stats* out) {
   const uint32_t* misc = ctx->host_rb + kRing * kRingStride;
   const size_t src_lds = size_t(ctx->W.desc_bytes) <= kDescLdsMax ? size_t(ctx->W.desc_bytes) : 0;
   const mmc_counters cnt_snap = ctx->cnt;
-  unsigned long long ev_snap[kEvN];
+  unsigned long long ev_snap[kEvWords];
   for (;;) {
     rc = ResetRun(ctx);
     if (rc) return rc;
