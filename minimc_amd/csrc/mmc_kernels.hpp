@@ -1055,8 +1055,8 @@ __global__ void __launch_bounds__(kThreads) KTslPod(DevWorld W, RunState R, int 
       p = R.QT[q];
       alive = TslPodEvent(W, R, p, H);
     }
-    uint32_t tot;
-    const uint32_t iq = WavePush(Ring(R, iter, kQ), alive, tot);
+    // one atomic per block and pass (BlockPush): 5.92 -> 5.41 ms per C5 cycle
+    const uint32_t iq = BlockPush(Ring(R, iter, kQ), alive);
     if (alive) R.Q[qin ^ 1][iq] = p;
     t_tsl.add(valid);
   }
