@@ -460,29 +460,60 @@ __device__ __forceinline__ TBracket PartTBracket(const double* H, const DTslPart
   const bool below = t_hi == 0;
   return {below ? t_hi : t_hi - 1, t_hi};
 }
-// BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256)
-__device__ __forceinline__ double PartEvaluateAt(const DevWorld& W, const double* H,
-                                                 const DTslPart& P, int cdf_i, int x_i,
-                                                 double T, TBracket br) {
+// The two rows of POD coefficients (grid point x_i at the bracket's two
+// temperatures) that every evaluation of one sample multiplies.  A sample
+// loads them once, into registers for ranks up to kRowRegs: the CDF
+// bisection then waits on the CDF modes in LDS only, not on a global load per
+// probe.  Larger ranks read the rows in memory at each evaluation.
+constexpr int kRowRegs = 8;
+struct PartRow {
+  TBracket br;
+  const double* xh;  // row at t_hi
+  const double* xl;  // row at t_lo
+  double h[kRowRegs], l[kRowRegs];
+};
+__device__ __forceinline__ PartRow LoadPartRow(const DevWorld& W, const DTslPart& P, int x_i,
+                                               TBracket br) {
+  PartRow r;
+  r.br = br;
+  r.xh = W.pool + P.xt_off + (x_i * P.nT + br.t_hi) * P.rank;
+  r.xl = W.pool + P.xt_off + (x_i * P.nT + br.t_lo) * P.rank;
+  const bool regs = P.rank <= kRowRegs;
+#pragma unroll
+  for (int o = 0; o < kRowRegs; o++) {
+    const bool in = regs && o < P.rank;
+    r.h[o] = in ? r.xh[o] : 0.0;
+    r.l[o] = in ? r.xl[o] : 0.0;
+  }
+  return r;
+}
+// BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256) at
+// CDF point cdf_i, for the grid point and temperature bracket of row r.  The
+// sums run over the modes in order, whichever copy of the rows they read.
+__device__ __forceinline__ double PartEvaluateAt(const double* H, const DTslPart& P, int cdf_i,
+                                                 double T, const PartRow& r) {
   const double* Ts = H + P.T_off;
-  const int t_hi = br.t_hi, t_lo = br.t_lo;
   const double* S = H + P.S_off;
   const double* cdf = H + P.cdf_off + cdf_i * P.rank;
-  const double* xh = W.pool + P.xt_off + (x_i * P.nT + t_hi) * P.rank;
-  const double* xl = W.pool + P.xt_off + (x_i * P.nT + t_lo) * P.rank;
   double v_hi = 0, v_lo = 0;
-  for (int o = 0; o < P.rank; o++) {
-    const double sc = S[o] * cdf[o];
-    v_hi = v_hi + sc * xh[o];
-    v_lo = v_lo + sc * xl[o];
+  if (P.rank <= kRowRegs) {
+#pragma unroll
+    for (int o = 0; o < kRowRegs; o++) {
+      if (o < P.rank) {
+        const double sc = S[o] * cdf[o];
+        v_hi = v_hi + sc * r.h[o];
+        v_lo = v_lo + sc * r.l[o];
+      }
+    }
+  } else {
+    for (int o = 0; o < P.rank; o++) {
+      const double sc = S[o] * cdf[o];
+      v_hi = v_hi + sc * r.xh[o];
+      v_lo = v_lo + sc * r.xl[o];
+    }
   }
-  const double T_hi = Ts[t_hi], T_lo = Ts[t_lo];
+  const double T_hi = Ts[r.br.t_hi], T_lo = Ts[r.br.t_lo];
   return v_lo + (v_hi - v_lo) / (T_hi - T_lo) * (T - T_lo);
-}
-__device__ __forceinline__ double PartEvaluate(const DevWorld& W, const double* H,
-                                               const DTslPart& P, int cdf_i, int x_i,
-                                               double T) {
-  return PartEvaluateAt(W, H, P, cdf_i, x_i, T, PartTBracket(H, P, T));
 }
 // ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
 __device__ __forceinline__ double TslTotal(const double* H, const DTsl& t, double E,
@@ -540,16 +571,15 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, c
   const int p_i = PartitionOf(H + t.E_ends_off, t.n_bp, e_s, local);
   const DTslPart P = W.parts[t.bp_begin + p_i];
   const double* F = H + P.F_off;
-  const TBracket br = PartTBracket(H, P, T);
+  const PartRow row = LoadPartRow(W, P, local, PartTBracket(H, P, T));
   for (int k = 0; k < kBetaResampleLimit; k++) {
     const double u = Canonical(g);
     const int f_hi = UpperBoundArr(F, P.nF, u);
     const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
     const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-    const double b_lo = f_hi != 0 ? PartEvaluateAt(W, H, P, f_hi - 1, local, T, br)
-                                  : -E_s / (kBoltzmann * T);
-    const double b_hi =
-        f_hi != P.nF ? PartEvaluateAt(W, H, P, f_hi, local, T, br) : t.beta_cutoff;
+    const double b_lo =
+        f_hi != 0 ? PartEvaluateAt(H, P, f_hi - 1, T, row) : -E_s / (kBoltzmann * T);
+    const double b_hi = f_hi != P.nF ? PartEvaluateAt(H, P, f_hi, T, row) : t.beta_cutoff;
     const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
     const double b_min = -E / (kBoltzmann * T);
     if (b_min <= b) {
@@ -566,9 +596,8 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, c
 // two bisections are independent and run in lockstep here so that their
 // memory round trips overlap.  Each follows exactly the probe sequence of a
 // single std::upper_bound.
-__device__ __forceinline__ void FindCdf2(const DevWorld& W, const double* H,
-                                         const DTslPart& P, int local, double T,
-                                         TBracket br, double a0, double a1,
+__device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, double T,
+                                         const PartRow& row, double a0, double a1,
                                          double alpha_cutoff, double& F0, double& F1) {
   const double* F = H + P.F_off;
   const int nF = P.nF;
@@ -576,8 +605,8 @@ __device__ __forceinline__ void FindCdf2(const DevWorld& W, const double* H,
   while (len0 > 0 || len1 > 0) {
     const int half0 = len0 >> 1, half1 = len1 >> 1;
     const int mid0 = first0 + half0, mid1 = first1 + half1;
-    const double v0 = len0 > 0 ? PartEvaluateAt(W, H, P, mid0, local, T, br) : 0;
-    const double v1 = len1 > 0 ? PartEvaluateAt(W, H, P, mid1, local, T, br) : 0;
+    const double v0 = len0 > 0 ? PartEvaluateAt(H, P, mid0, T, row) : 0;
+    const double v1 = len1 > 0 ? PartEvaluateAt(H, P, mid1, T, row) : 0;
     if (len0 > 0) {
       if (a0 < v0) {
         len0 = half0;
@@ -596,10 +625,10 @@ __device__ __forceinline__ void FindCdf2(const DevWorld& W, const double* H,
     }
   }
   const int h0 = first0, h1 = first1;
-  const double a_lo0 = h0 != 0 ? PartEvaluateAt(W, H, P, h0 - 1, local, T, br) : 0;
-  const double a_hi0 = h0 != nF ? PartEvaluateAt(W, H, P, h0, local, T, br) : alpha_cutoff;
-  const double a_lo1 = h1 != 0 ? PartEvaluateAt(W, H, P, h1 - 1, local, T, br) : 0;
-  const double a_hi1 = h1 != nF ? PartEvaluateAt(W, H, P, h1, local, T, br) : alpha_cutoff;
+  const double a_lo0 = h0 != 0 ? PartEvaluateAt(H, P, h0 - 1, T, row) : 0;
+  const double a_hi0 = h0 != nF ? PartEvaluateAt(H, P, h0, T, row) : alpha_cutoff;
+  const double a_lo1 = h1 != 0 ? PartEvaluateAt(H, P, h1 - 1, T, row) : 0;
+  const double a_hi1 = h1 != nF ? PartEvaluateAt(H, P, h1, T, row) : alpha_cutoff;
   const double Fl0 = h0 != 0 ? F[h0 - 1] : 0, Fh0 = h0 != nF ? F[h0] : 1;
   const double Fl1 = h1 != 0 ? F[h1 - 1] : 0, Fh1 = h1 != nF ? F[h1] : 1;
   F0 = Fl0 + (a0 - a_lo0) * (Fh0 - Fl0) / (a_hi0 - a_lo0);
@@ -641,16 +670,16 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const double*
   const DTslPart P = W.parts[t.ap_begin + p_i];
   const double* F = H + P.F_off;
   const int nF = P.nF;
-  const TBracket br = PartTBracket(H, P, T);
+  const PartRow row = LoadPartRow(W, P, local, PartTBracket(H, P, T));
   double F_min, F_max;
-  FindCdf2(W, H, P, local, T, br, a_min, a_max, t.alpha_cutoff, F_min, F_max);
+  FindCdf2(H, P, T, row, a_min, a_max, t.alpha_cutoff, F_min, F_max);
   for (int k = 0; k < kAlphaResampleLimit; k++) {
     const double u = F_min + Canonical(g) * (F_max - F_min);
     const int hi = UpperBoundArr(F, nF, u);
     const double F_lo = hi != 0 ? F[hi - 1] : 0;
     const double F_hi = hi != nF ? F[hi] : 1;
-    const double a_lo = hi != 0 ? PartEvaluateAt(W, H, P, hi - 1, local, T, br) : 0;
-    const double a_hi = hi != nF ? PartEvaluateAt(W, H, P, hi, local, T, br) : t.alpha_cutoff;
+    const double a_lo = hi != 0 ? PartEvaluateAt(H, P, hi - 1, T, row) : 0;
+    const double a_hi = hi != nF ? PartEvaluateAt(H, P, hi, T, row) : t.alpha_cutoff;
     const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
     if (a_min < a && a < a_max) {
       const double b_root = sqrt(E + b * kBoltzmann * T);
