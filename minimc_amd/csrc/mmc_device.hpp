@@ -462,9 +462,11 @@ __device__ __forceinline__ TBracket PartTBracket(const double* H, const DTslPart
 }
 // The two rows of POD coefficients (grid point x_i at the bracket's two
 // temperatures) that every evaluation of one sample multiplies.  A sample
-// loads them once, into registers for ranks up to kRowRegs: the CDF
+// loads them once, into registers when the rank is kRowRegs: the CDF
 // bisection then waits on the CDF modes in LDS only, not on a global load per
-// probe.  Larger ranks read the rows in memory at each evaluation.
+// probe, and the sums are unrolled with no branch per mode, so that an
+// evaluation's LDS reads issue together.  Other ranks read the rows in
+// memory at each evaluation.
 constexpr int kRowRegs = 8;
 struct PartRow {
   TBracket br;
@@ -478,12 +480,13 @@ __device__ __forceinline__ PartRow LoadPartRow(const DevWorld& W, const DTslPart
   r.br = br;
   r.xh = W.pool + P.xt_off + (x_i * P.nT + br.t_hi) * P.rank;
   r.xl = W.pool + P.xt_off + (x_i * P.nT + br.t_lo) * P.rank;
-  const bool regs = P.rank <= kRowRegs;
+  // (other ranks load mode 0 into every register, in bounds and unused: no
+  // branch, which keeps the rows in registers)
+  const bool regs = P.rank == kRowRegs;
 #pragma unroll
   for (int o = 0; o < kRowRegs; o++) {
-    const bool in = regs && o < P.rank;
-    r.h[o] = in ? r.xh[o] : 0.0;
-    r.l[o] = in ? r.xl[o] : 0.0;
+    r.h[o] = r.xh[regs ? o : 0];
+    r.l[o] = r.xl[regs ? o : 0];
   }
   return r;
 }
@@ -496,14 +499,12 @@ __device__ __forceinline__ double PartEvaluateAt(const double* H, const DTslPart
   const double* S = H + P.S_off;
   const double* cdf = H + P.cdf_off + cdf_i * P.rank;
   double v_hi = 0, v_lo = 0;
-  if (P.rank <= kRowRegs) {
+  if (P.rank == kRowRegs) {
 #pragma unroll
     for (int o = 0; o < kRowRegs; o++) {
-      if (o < P.rank) {
-        const double sc = S[o] * cdf[o];
-        v_hi = v_hi + sc * r.h[o];
-        v_lo = v_lo + sc * r.l[o];
-      }
+      const double sc = S[o] * cdf[o];
+      v_hi = v_hi + sc * r.h[o];
+      v_lo = v_lo + sc * r.l[o];
     }
   } else {
     for (int o = 0; o < P.rank; o++) {
@@ -514,6 +515,18 @@ __device__ __forceinline__ double PartEvaluateAt(const double* H, const DTslPart
   }
   const double T_hi = Ts[r.br.t_hi], T_lo = Ts[r.br.t_lo];
   return v_lo + (v_hi - v_lo) / (T_hi - T_lo) * (T - T_lo);
+}
+// The values at the two ends of CDF interval h of an axis of nF > 0 points:
+// lo at h - 1 (fallback lo_out when h == 0) and hi at h (fallback hi_out when
+// h == nF).  Both evaluations run at in-range indices whatever h is, so their
+// reads and sums overlap; the fallbacks replace what they computed.
+__device__ __forceinline__ void PartEnds(const double* H, const DTslPart& P, int h, double T,
+                                         const PartRow& r, double lo_out, double hi_out,
+                                         double& lo, double& hi) {
+  const double e_lo = PartEvaluateAt(H, P, h != 0 ? h - 1 : 0, T, r);
+  const double e_hi = PartEvaluateAt(H, P, h != P.nF ? h : P.nF - 1, T, r);
+  lo = h != 0 ? e_lo : lo_out;
+  hi = h != P.nF ? e_hi : hi_out;
 }
 // ThermalScattering::GetTotal (ThermalScattering.cpp:111-157)
 __device__ __forceinline__ double TslTotal(const double* H, const DTsl& t, double E,
@@ -577,9 +590,8 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, c
     const int f_hi = UpperBoundArr(F, P.nF, u);
     const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
     const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-    const double b_lo =
-        f_hi != 0 ? PartEvaluateAt(H, P, f_hi - 1, T, row) : -E_s / (kBoltzmann * T);
-    const double b_hi = f_hi != P.nF ? PartEvaluateAt(H, P, f_hi, T, row) : t.beta_cutoff;
+    double b_lo, b_hi;
+    PartEnds(H, P, f_hi, T, row, -E_s / (kBoltzmann * T), t.beta_cutoff, b_lo, b_hi);
     const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
     const double b_min = -E / (kBoltzmann * T);
     if (b_min <= b) {
@@ -605,8 +617,10 @@ __device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, dou
   while (len0 > 0 || len1 > 0) {
     const int half0 = len0 >> 1, half1 = len1 >> 1;
     const int mid0 = first0 + half0, mid1 = first1 + half1;
-    const double v0 = len0 > 0 ? PartEvaluateAt(H, P, mid0, T, row) : 0;
-    const double v1 = len1 > 0 ? PartEvaluateAt(H, P, mid1, T, row) : 0;
+    // both probes run, at an index inside the axis, whether or not their
+    // search is still open, so that their reads and sums overlap
+    const double v0 = PartEvaluateAt(H, P, len0 > 0 ? mid0 : 0, T, row);
+    const double v1 = PartEvaluateAt(H, P, len1 > 0 ? mid1 : 0, T, row);
     if (len0 > 0) {
       if (a0 < v0) {
         len0 = half0;
@@ -625,10 +639,9 @@ __device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, dou
     }
   }
   const int h0 = first0, h1 = first1;
-  const double a_lo0 = h0 != 0 ? PartEvaluateAt(H, P, h0 - 1, T, row) : 0;
-  const double a_hi0 = h0 != nF ? PartEvaluateAt(H, P, h0, T, row) : alpha_cutoff;
-  const double a_lo1 = h1 != 0 ? PartEvaluateAt(H, P, h1 - 1, T, row) : 0;
-  const double a_hi1 = h1 != nF ? PartEvaluateAt(H, P, h1, T, row) : alpha_cutoff;
+  double a_lo0, a_hi0, a_lo1, a_hi1;
+  PartEnds(H, P, h0, T, row, 0, alpha_cutoff, a_lo0, a_hi0);
+  PartEnds(H, P, h1, T, row, 0, alpha_cutoff, a_lo1, a_hi1);
   const double Fl0 = h0 != 0 ? F[h0 - 1] : 0, Fh0 = h0 != nF ? F[h0] : 1;
   const double Fl1 = h1 != 0 ? F[h1 - 1] : 0, Fh1 = h1 != nF ? F[h1] : 1;
   F0 = Fl0 + (a0 - a_lo0) * (Fh0 - Fl0) / (a_hi0 - a_lo0);
@@ -678,8 +691,8 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const double*
     const int hi = UpperBoundArr(F, nF, u);
     const double F_lo = hi != 0 ? F[hi - 1] : 0;
     const double F_hi = hi != nF ? F[hi] : 1;
-    const double a_lo = hi != 0 ? PartEvaluateAt(H, P, hi - 1, T, row) : 0;
-    const double a_hi = hi != nF ? PartEvaluateAt(H, P, hi, T, row) : t.alpha_cutoff;
+    double a_lo, a_hi;
+    PartEnds(H, P, hi, T, row, 0, t.alpha_cutoff, a_lo, a_hi);
     const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
     if (a_min < a && a < a_max) {
       const double b_root = sqrt(E + b * kBoltzmann * T);

@@ -548,12 +548,16 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       t.T_modes_off = F.AddTot(d.total_T, size_t(d.n_T) * d.rank);
       t.S_off = F.AddTot(d.total_S, d.rank);
       t.rank = d.rank;
+      // every partition has points on each axis: the samplers evaluate the
+      // ends of a CDF interval at in-range indices
+      bool bad_part = false;
       auto add_parts = [&](const mmc_tsl_partition* p, int np, int& begin, int& xs_off,
                            int& nxs, int& ends_off) {
         begin = int(parts.size());
         std::vector<double> xs;
         std::vector<int> ends;
         for (int k = 0; k < np; k++) {
+          if (p[k].n_F < 1 || p[k].n_x < 1 || p[k].n_T < 1 || p[k].rank < 1) bad_part = true;
           DTslPart q{};
           q.F_off = F.AddHot(p[k].F, p[k].n_F);
           q.nF = p[k].n_F;
@@ -580,6 +584,7 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
       add_parts(d.alpha_partitions, d.n_alpha_partitions, t.ap_begin, t.betas_off,
                 t.nbetas, t.beta_ends_off);
       t.n_ap = d.n_alpha_partitions;
+      if (bad_part) return Fail(ctx, MMC_ERR_ARG, "bad tsl partition (an empty axis)");
       t.beta_cutoff = d.beta_cutoff;
       t.alpha_cutoff = d.alpha_cutoff;
       t.cutoff_energy = d.total_E_axis[d.n_E - 1];  // ThermalScattering.hpp:125-126
