@@ -76,6 +76,7 @@ struct DGrid {
   int x_off, n;         // keys in pool
   int hash_off, n_hash; // bucket bounds in ipool (n_hash + 1 entries), 0 = none
   double log_x0, inv_dlog;
+  double x_first, x_last;  // x[0] and x[n - 1], kept with the descriptor
 };
 struct DCeReaction {
   int kind, tsl, xs, nubar;  // xs / nubar: table index (-1 none)
@@ -408,7 +409,7 @@ __device__ __forceinline__ int GridUpperBound(const DevWorld& W, const DGrid& g,
                                               const int* hash, double key) {
   const double* x = W.pool + g.x_off;
   int lo = 0, hi = g.n;
-  if (g.n_hash > 0 && key > x[0] && key < x[g.n - 1]) {
+  if (g.n_hash > 0 && key > g.x_first && key < g.x_last) {
     const double f = (log(key) - g.log_x0) * g.inv_dlog;
     int b = int(f);
     b = b < 0 ? 0 : (b >= g.n_hash ? g.n_hash - 1 : b);
@@ -493,13 +494,16 @@ __device__ __forceinline__ PartRow LoadPartRow(const DevWorld& W, const DTslPart
 // BetaPartition/AlphaPartition::Evaluate (ThermalScattering.cpp:183-256) at
 // CDF point cdf_i, for the grid point and temperature bracket of row r.  The
 // sums run over the modes in order, whichever copy of the rows they read.
+// kRegs: the partition's rank is kRowRegs (the caller checked it once per
+// sample), so that a sampler's evaluations hold no branch and overlap.
+template <bool kRegs>
 __device__ __forceinline__ double PartEvaluateAt(const double* H, const DTslPart& P, int cdf_i,
                                                  double T, const PartRow& r) {
   const double* Ts = H + P.T_off;
   const double* S = H + P.S_off;
   const double* cdf = H + P.cdf_off + cdf_i * P.rank;
   double v_hi = 0, v_lo = 0;
-  if (P.rank == kRowRegs) {
+  if constexpr (kRegs) {
 #pragma unroll
     for (int o = 0; o < kRowRegs; o++) {
       const double sc = S[o] * cdf[o];
@@ -520,11 +524,12 @@ __device__ __forceinline__ double PartEvaluateAt(const double* H, const DTslPart
 // lo at h - 1 (fallback lo_out when h == 0) and hi at h (fallback hi_out when
 // h == nF).  Both evaluations run at in-range indices whatever h is, so their
 // reads and sums overlap; the fallbacks replace what they computed.
+template <bool kRegs>
 __device__ __forceinline__ void PartEnds(const double* H, const DTslPart& P, int h, double T,
                                          const PartRow& r, double lo_out, double hi_out,
                                          double& lo, double& hi) {
-  const double e_lo = PartEvaluateAt(H, P, h != 0 ? h - 1 : 0, T, r);
-  const double e_hi = PartEvaluateAt(H, P, h != P.nF ? h : P.nF - 1, T, r);
+  const double e_lo = PartEvaluateAt<kRegs>(H, P, h != 0 ? h - 1 : 0, T, r);
+  const double e_hi = PartEvaluateAt<kRegs>(H, P, h != P.nF ? h : P.nF - 1, T, r);
   lo = h != 0 ? e_lo : lo_out;
   hi = h != P.nF ? e_hi : hi_out;
 }
@@ -545,13 +550,26 @@ __device__ __forceinline__ double TslTotal(const double* H, const DTsl& t, doubl
   const double* Em = H + t.E_modes_off;
   const double* Tm = H + t.T_modes_off;
   double ll = 0, lh = 0, hl = 0, hh = 0;
-  for (int o = 0; o < t.rank; o++) {
-    const double sel = S[o] * Em[e_lo * t.rank + o];
-    const double seh = S[o] * Em[e_hi * t.rank + o];
-    ll = ll + sel * Tm[t_lo * t.rank + o];
-    lh = lh + sel * Tm[t_hi * t.rank + o];
-    hl = hl + seh * Tm[t_lo * t.rank + o];
-    hh = hh + seh * Tm[t_hi * t.rank + o];
+  if (t.rank == kRowRegs) {
+    // (unrolled with no branch per mode: the LDS reads issue together)
+#pragma unroll
+    for (int o = 0; o < kRowRegs; o++) {
+      const double sel = S[o] * Em[e_lo * kRowRegs + o];
+      const double seh = S[o] * Em[e_hi * kRowRegs + o];
+      ll = ll + sel * Tm[t_lo * kRowRegs + o];
+      lh = lh + sel * Tm[t_hi * kRowRegs + o];
+      hl = hl + seh * Tm[t_lo * kRowRegs + o];
+      hh = hh + seh * Tm[t_hi * kRowRegs + o];
+    }
+  } else {
+    for (int o = 0; o < t.rank; o++) {
+      const double sel = S[o] * Em[e_lo * t.rank + o];
+      const double seh = S[o] * Em[e_hi * t.rank + o];
+      ll = ll + sel * Tm[t_lo * t.rank + o];
+      lh = lh + sel * Tm[t_hi * t.rank + o];
+      hl = hl + seh * Tm[t_lo * t.rank + o];
+      hh = hh + seh * Tm[t_hi * t.rank + o];
+    }
   }
   const double E_lo = Ex[e_lo], E_hi = Ex[e_hi];
   const double r_E = below_E ? 1 : (E - E_lo) / (E_hi - E_lo);
@@ -571,6 +589,28 @@ __device__ __forceinline__ int PartitionOf(const double* ends, int n_parts, int 
   local = p_i == 0 ? idx : idx - int(ends[p_i - 1]);
   return p_i;
 }
+// The draws of SampleBeta once the partition and its rows are known.
+template <bool kRegs>
+__device__ __forceinline__ bool BetaDraws(const double* H, const DTsl& t, const DTslPart& P,
+                                          const PartRow& row, uint32_t& g, double E, double E_s,
+                                          double T, double& beta) {
+  const double* F = H + P.F_off;
+  for (int k = 0; k < kBetaResampleLimit; k++) {
+    const double u = Canonical(g);
+    const int f_hi = UpperBoundArr(F, P.nF, u);
+    const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
+    const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
+    double b_lo, b_hi;
+    PartEnds<kRegs>(H, P, f_hi, T, row, -E_s / (kBoltzmann * T), t.beta_cutoff, b_lo, b_hi);
+    const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
+    const double b_min = -E / (kBoltzmann * T);
+    if (b_min <= b) {
+      beta = b;
+      return true;
+    }
+  }
+  return false;
+}
 // H: the hot block (DevWorld::hot or its copy in LDS).
 __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, const DTsl& t,
                                            uint32_t& g, double E, double T, double& beta) {
@@ -583,23 +623,9 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, c
   int local;
   const int p_i = PartitionOf(H + t.E_ends_off, t.n_bp, e_s, local);
   const DTslPart P = W.parts[t.bp_begin + p_i];
-  const double* F = H + P.F_off;
   const PartRow row = LoadPartRow(W, P, local, PartTBracket(H, P, T));
-  for (int k = 0; k < kBetaResampleLimit; k++) {
-    const double u = Canonical(g);
-    const int f_hi = UpperBoundArr(F, P.nF, u);
-    const double F_lo = f_hi != 0 ? F[f_hi - 1] : 0;
-    const double F_hi = f_hi != P.nF ? F[f_hi] : 1;
-    double b_lo, b_hi;
-    PartEnds(H, P, f_hi, T, row, -E_s / (kBoltzmann * T), t.beta_cutoff, b_lo, b_hi);
-    const double b = b_lo + (u - F_lo) / (F_hi - F_lo) * (b_hi - b_lo);
-    const double b_min = -E / (kBoltzmann * T);
-    if (b_min <= b) {
-      beta = b;
-      return true;
-    }
-  }
-  return false;
+  return P.rank == kRowRegs ? BetaDraws<true>(H, t, P, row, g, E, E_s, T, beta)
+                            : BetaDraws<false>(H, t, P, row, g, E, E_s, T, beta);
 }
 // find_cdf of SampleAlpha (ThermalScattering.cpp:395-418): upper_bound over
 // the CDF axis with a POD evaluation inside the comparator (the reconstructed
@@ -608,6 +634,7 @@ __device__ __forceinline__ bool SampleBeta(const DevWorld& W, const double* H, c
 // two bisections are independent and run in lockstep here so that their
 // memory round trips overlap.  Each follows exactly the probe sequence of a
 // single std::upper_bound.
+template <bool kRegs>
 __device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, double T,
                                          const PartRow& row, double a0, double a1,
                                          double alpha_cutoff, double& F0, double& F1) {
@@ -619,8 +646,8 @@ __device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, dou
     const int mid0 = first0 + half0, mid1 = first1 + half1;
     // both probes run, at an index inside the axis, whether or not their
     // search is still open, so that their reads and sums overlap
-    const double v0 = PartEvaluateAt(H, P, len0 > 0 ? mid0 : 0, T, row);
-    const double v1 = PartEvaluateAt(H, P, len1 > 0 ? mid1 : 0, T, row);
+    const double v0 = PartEvaluateAt<kRegs>(H, P, len0 > 0 ? mid0 : 0, T, row);
+    const double v1 = PartEvaluateAt<kRegs>(H, P, len1 > 0 ? mid1 : 0, T, row);
     if (len0 > 0) {
       if (a0 < v0) {
         len0 = half0;
@@ -640,14 +667,45 @@ __device__ __forceinline__ void FindCdf2(const double* H, const DTslPart& P, dou
   }
   const int h0 = first0, h1 = first1;
   double a_lo0, a_hi0, a_lo1, a_hi1;
-  PartEnds(H, P, h0, T, row, 0, alpha_cutoff, a_lo0, a_hi0);
-  PartEnds(H, P, h1, T, row, 0, alpha_cutoff, a_lo1, a_hi1);
+  PartEnds<kRegs>(H, P, h0, T, row, 0, alpha_cutoff, a_lo0, a_hi0);
+  PartEnds<kRegs>(H, P, h1, T, row, 0, alpha_cutoff, a_lo1, a_hi1);
   const double Fl0 = h0 != 0 ? F[h0 - 1] : 0, Fh0 = h0 != nF ? F[h0] : 1;
   const double Fl1 = h1 != 0 ? F[h1 - 1] : 0, Fh1 = h1 != nF ? F[h1] : 1;
   F0 = Fl0 + (a0 - a_lo0) * (Fh0 - Fl0) / (a_hi0 - a_lo0);
   F1 = Fl1 + (a1 - a_lo1) * (Fh1 - Fl1) / (a_hi1 - a_lo1);
 }
 
+// The CDF bounds and draws of SampleAlpha once the partition and its rows
+// are known.
+template <bool kRegs>
+__device__ __forceinline__ uint32_t AlphaDraws(const double* H, const DTsl& t,
+                                               const DTslPart& P, const PartRow& row,
+                                               uint32_t& g, double b, double E, double T,
+                                               double sqrt_E, double akT, double a_min,
+                                               double a_max, double& alpha) {
+  const double* F = H + P.F_off;
+  const int nF = P.nF;
+  double F_min, F_max;
+  FindCdf2<kRegs>(H, P, T, row, a_min, a_max, t.alpha_cutoff, F_min, F_max);
+  for (int k = 0; k < kAlphaResampleLimit; k++) {
+    const double u = F_min + Canonical(g) * (F_max - F_min);
+    const int hi = UpperBoundArr(F, nF, u);
+    const double F_lo = hi != 0 ? F[hi - 1] : 0;
+    const double F_hi = hi != nF ? F[hi] : 1;
+    double a_lo, a_hi;
+    PartEnds<kRegs>(H, P, hi, T, row, 0, t.alpha_cutoff, a_lo, a_hi);
+    const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
+    if (a_min < a && a < a_max) {
+      const double b_root = sqrt(E + b * kBoltzmann * T);
+      const double em = sqrt_E - b_root, ep = sqrt_E + b_root;
+      const double ba_min = em * em / akT;
+      const double ba_max = ep * ep / akT;
+      alpha = ba_min + (a - a_min) * (ba_max - ba_min) / (a_max - a_min);
+      return 0;
+    }
+  }
+  return kErrResample;
+}
 // ThermalScattering::SampleAlpha (ThermalScattering.cpp:340-463), quirk Q2
 // (operator precedence at :361-365) reproduced.
 __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const double* H,
@@ -681,29 +739,10 @@ __device__ __forceinline__ uint32_t SampleAlpha(const DevWorld& W, const double*
   int local;
   const int p_i = PartitionOf(H + t.beta_ends_off, t.n_ap, b_s_i, local);
   const DTslPart P = W.parts[t.ap_begin + p_i];
-  const double* F = H + P.F_off;
-  const int nF = P.nF;
   const PartRow row = LoadPartRow(W, P, local, PartTBracket(H, P, T));
-  double F_min, F_max;
-  FindCdf2(H, P, T, row, a_min, a_max, t.alpha_cutoff, F_min, F_max);
-  for (int k = 0; k < kAlphaResampleLimit; k++) {
-    const double u = F_min + Canonical(g) * (F_max - F_min);
-    const int hi = UpperBoundArr(F, nF, u);
-    const double F_lo = hi != 0 ? F[hi - 1] : 0;
-    const double F_hi = hi != nF ? F[hi] : 1;
-    double a_lo, a_hi;
-    PartEnds(H, P, hi, T, row, 0, t.alpha_cutoff, a_lo, a_hi);
-    const double a = a_lo + (u - F_lo) / (F_hi - F_lo) * (a_hi - a_lo);
-    if (a_min < a && a < a_max) {
-      const double b_root = sqrt(E + b * kBoltzmann * T);
-      const double em = sqrt_E - b_root, ep = sqrt_E + b_root;
-      const double ba_min = em * em / akT;
-      const double ba_max = ep * ep / akT;
-      alpha = ba_min + (a - a_min) * (ba_max - ba_min) / (a_max - a_min);
-      return 0;
-    }
-  }
-  return kErrResample;
+  return P.rank == kRowRegs
+             ? AlphaDraws<true>(H, t, P, row, g, b, E, T, sqrt_E, akT, a_min, a_max, alpha)
+             : AlphaDraws<false>(H, t, P, row, g, b, E, T, sqrt_E, akT, a_min, a_max, alpha);
 }
 
 // ---- continuous-energy cross sections --------------------------------------

@@ -170,6 +170,8 @@ struct Flat {
     g.n = n;
     g.n_hash = 0;
     g.hash_off = 0;
+    g.x_first = n > 0 ? x[0] : 0.0;
+    g.x_last = n > 0 ? x[n - 1] : 0.0;
     if (n >= 64 && x[0] > 0) {
       // log-spaced buckets; H[b] = upper_bound(x, edge_b) (host, exact)
       int nh = 1;
