@@ -770,6 +770,10 @@ struct XsMemo {
   int grid = -1, gi = 0;
   double E = 0;
   int hint_grid = -1, hint_gi = 0;
+  // hint_exact: the hint is the grid index of this very energy (the caller
+  // knows it: a collision follows the advance that computed it, and nothing
+  // between them changes the energy), so it is taken without reading the grid
+  bool hint_exact = false;
   int tsl = -1;
   double tE = 0, tT = 0, tval = 0;
   const double* tot = nullptr;  // DevWorld::tot or its copy in LDS (never null in use)
@@ -787,7 +791,9 @@ __device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNucl
   if (memo && memo->hint_grid == n.grid) {
     const double* x = W.pool + g.x_off;
     const int i = memo->hint_gi;
-    if (i >= 0 && i <= g.n && (i == 0 || x[i - 1] <= E) && (i == g.n || x[i] > E)) gi = i;
+    if (i >= 0 && i <= g.n &&
+        (memo->hint_exact || ((i == 0 || x[i - 1] <= E) && (i == g.n || x[i] > E))))
+      gi = i;
   }
   // lds_hash is the LDS copy of grid 0's hash; other grids use their own
   if (gi < 0)
@@ -802,6 +808,7 @@ __device__ __forceinline__ int NuclideGridIndex(const DevWorld& W, const DCeNucl
 __device__ __forceinline__ double TabAt(const DevWorld& W, int t, int gi, double E) {
   return gi >= 0 ? TableAtI(W, t, gi, E) : TableAt(W, t, E);
 }
+
 // ContinuousScatter::GetCrossSection / GetMajorant (ContinuousReaction.cpp:77-117)
 __device__ __forceinline__ double RxXs(const DevWorld& W, const DCeNuclide& n,
                                        const DCeReaction& r, int gi, double E, double T,

@@ -927,6 +927,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   XsMemo memo;
   memo.hint_grid = R.B.xs_grid[p];
   memo.hint_gi = R.B.xs_gi[p];
+  memo.hint_exact = true;  // written by the advance that queued this collision
   memo.tot = Ht;
   // the collision estimator of k: +0 increments (no fissile nuclide here) are
   // skipped, as for the track-length sum, and so is the total they would

@@ -105,3 +105,18 @@ def test_tsl_index_out_of_range_is_an_error(pair):
     for bad in (-1, len(pr.tsl)):
         with pytest.raises(MmcError, match="no such tsl table"):
             t.tsl_sample_pod(E, T, r, a, b, tsl=bad)
+
+
+def test_partition_with_an_empty_axis_is_refused():
+    """A thermal-scattering partition with no CDF points is refused when the
+    World is built: the samplers evaluate both ends of a CDF interval at
+    in-range indices of the partition's axes."""
+    require_gpu()
+    import copy
+    from minimc_amd.tracker import MmcError, Tracker
+    t = copy.deepcopy(configs.tsl_tables())
+    p = t["beta_partitions"][0]
+    p["F"] = np.zeros(0)
+    p["cdf_modes"] = np.zeros((0, len(p["S"])))
+    with pytest.raises(MmcError, match="bad tsl partition"):
+        Tracker(configs.tsl_box(t, T=600.0), max_in_flight=1 << 10)
