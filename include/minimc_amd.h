@@ -153,7 +153,9 @@ typedef struct mmc_ce_nuclide {
  * with x = incident energy E for beta partitions and x = beta for alpha
  * partitions (BetaPartition/AlphaPartition::Evaluate,
  * ThermalScattering.cpp:183-256).  Layout is the reference's pandas-HDF5
- * block layout: row-major, last axis fastest (HDF5DataSet.hpp:118-167). */
+ * block layout: row-major, last axis fastest (HDF5DataSet.hpp:118-167).
+ * Every count is at least 1: mmc_create refuses a partition with an empty
+ * axis (MMC_ERR_ARG). */
 typedef struct mmc_tsl_partition {
   int32_t n_F, n_x, n_T, rank;
   const double* F;         /* [n_F]            CDF axis                 */
