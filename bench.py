@@ -56,12 +56,18 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 BYTES = {
     "c2": {"advance": 108 + 32 + 2 * 8,        # total xs + 2 planes
            "collide": 112 + 3 * 32,            # total, capture, elastic
+           "collide_fused": 36 + 3 * 32,
            "cross": 24, "tsl": 0},
     "c5": {"advance": 108 + 32 + 6 * 8,        # total xs + 6 cube faces
            "collide": 112 + 3 * 32,            # total, reaction split, elastic
+           "collide_fused": 36 + 3 * 32,
            "cross": 4,                         # no current estimators
            "tsl": 84 + 4 * 128},
 }
+# collide_fused: a collision done by the advance kernel right after the flight
+# (k-eigenvalue runs): its table reads and the 36 B of direction, energy and
+# random state it writes; the 76 B of particle state a separate collision
+# kernel reads are the advance's own writes, not counted twice.
 MLP_FLOP_PER_SAMPLE = 2 * (4 * 256 + 256 * 256 + 256 * 256 + 256 * 2)
 
 
@@ -194,6 +200,12 @@ def _roofline(cfg, kt, counts):
     nbytes = counts[dom] * BYTES[cfg][dom]
     if dom == "advance":
         nbytes += counts["cross"] * BYTES[cfg]["cross"]
+    # the advance kernel collides its particles itself when no collision
+    # kernel ran
+    fused = (dom == "advance" and counts.get("collide", 0) > 0 and
+             kt.get("collide", (0.0, 0))[1] == 0)
+    if fused:
+        nbytes += counts["collide"] * BYTES[cfg]["collide_fused"]
     achieved = nbytes / (ms * 1e-3) / 1e9
     traffic = _profiled_traffic(dom, "c5_multizone") if cfg == "c5" else None
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -201,7 +213,10 @@ def _roofline(cfg, kt, counts):
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE, "
                             "profiles/c5_multizone_hbm_traffic.md)" if traffic else None,
             "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
-            "bytes_per_event": BYTES[cfg][dom], "avg_launch_ms": ms / max(launches, 1)}
+            "bytes_per_event": BYTES[cfg][dom],
+            "collisions_in_kernel": fused,
+            "bytes_per_collision": BYTES[cfg]["collide_fused"] if fused else None,
+            "avg_launch_ms": ms / max(launches, 1)}
 
 
 def run_c2(args):

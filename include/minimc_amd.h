@@ -448,10 +448,14 @@ int mmc_get_counters(mmc_ctx* ctx, mmc_counters* out);
 enum mmc_kernel_kind {
   MMC_K_ADVANCE = 0, /* XS lookup + collision distance + CSG distance,
                         and the surface crossing that follows (cell
-                        search + current tally)                        */
+                        search + current tally); in k-eigenvalue runs
+                        with the tables in LDS also the collision that
+                        ends the flight                                */
   MMC_K_CROSS = 1,   /* reserved: crossings run in the advance kernel,
                         so this kind stays at 0 launches               */
-  MMC_K_COLLIDE = 2, /* nuclide + reaction sampling, collision physics */
+  MMC_K_COLLIDE = 2, /* nuclide + reaction sampling, collision physics
+                        (its own kernel where MMC_K_ADVANCE does not
+                        collide)                                       */
   MMC_K_TSL = 3,     /* thermal-scattering (alpha, beta) sampling: the
                         POD sampler, or the surrogate's gather + apply */
   MMC_K_MLP = 4,     /* the surrogate's fused BF16-MFMA network        */

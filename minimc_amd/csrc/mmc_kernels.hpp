@@ -993,6 +993,69 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
   t_col.flush(Ev(R, kEvCollide));
 }
 
+
+// The advance and, where the flight ends in a real collision, the collision
+// of the same particle, in one kernel: KAdvance's work, then KCollide's on
+// the slot the advance just wrote (bank D when the kernel compacts).  In
+// C5 nine flights in ten end in a collision, so few lanes idle through the
+// collision code, and the launch, the collision queue and the collision
+// kernel's ramp go away.  k-eigenvalue iterations only: there a fission banks
+// sites, while a fixed-source child claims a bank slot from misc[3], which
+// block 0 of this kernel sets.  Queues: alive after the flight (virtual
+// collision, crossing) -> Q (kQ); after the collision: thermal scattering ->
+// QT (kT), alive -> Q2 (kQ2); kC stays 0.
+template <bool kMove>
+__global__ void __launch_bounds__(256, 2) KAdvCol(DevWorld Wg, RunState R, int iter, int qin,
+                                                   DevBank D, uint32_t* fb) {
+  extern __shared__ int lds_hash[];
+  if (blockIdx.x == 0 && threadIdx.x < kRingStride) *Ring(R, iter + 1, threadIdx.x) = 0;
+  uint32_t n1;
+  const uint32_t n = LiveLength(R, iter - 1, n1);
+  const uint32_t nf = kMove && fb ? R.misc[kMiscFallback] : 0u;
+  if (kMove && blockIdx.x == 0 && threadIdx.x == 0) R.misc[3] = n + nf;  // slots in use
+  if (blockIdx.x * blockDim.x >= n + nf) return;  // no work: skip the staging too
+  const int* hash = StageHash(Wg, lds_hash);
+  double* lds_tot = reinterpret_cast<double*>(lds_hash + HashLdsInts(Wg));
+  const double* Ht = StageHot(Wg.tot, lds_tot, Wg.tot_n);
+  const DevWorld W = DescInLds(Wg, reinterpret_cast<unsigned char*>(lds_tot + Wg.tot_n));
+  RunState RD = R;  // the collision reads the bank the advance wrote
+  if (kMove) RD.B = D;
+  WaveTally t_adv, t_virt, t_cross, t_col;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n + nf;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    const bool valid = q < n;
+    AdvanceOut o{false, false, false, false};
+    uint32_t p = 0;
+    if (valid) {
+      p = LiveAt(R, qin, n1, q);
+      o = AdvanceEvent<true, kMove>(W, R, p, hash, Ht, D, q);
+    } else if (kMove && q < n + nf) {
+      const uint32_t f = fb[q - n];
+      MoveParticle(R.B, f, D, q);
+      MoveIndirect(W.n_pert, R.capacity, R.B, f, D, q);
+      fb[q - n] = q;
+    }
+    const uint32_t slot = kMove ? q : p;  // the particle's slot after this kernel
+    const bool live = o.to_q || o.x_alive;
+    bool alive = false, to_t = false;
+    if (o.to_c) CollideEvent(W, RD, slot, hash, Ht, nullptr, nullptr, alive, to_t);
+    const uint32_t iq = BlockPush(Ring(R, iter, kQ), live);
+    if (live) R.Q[qin ^ 1][iq] = slot;
+    uint32_t it, i2;
+    BlockPush2(Ring(R, iter, kT), to_t, alive, it, i2);
+    if (to_t) R.QT[it] = slot;
+    if (alive) R.Q2[qin ^ 1][i2] = slot;
+    t_adv.add(valid);
+    t_virt.add(o.to_q);
+    t_cross.add(o.to_x);
+    t_col.add(o.to_c);
+  }
+  t_adv.flush(Ev(R, kEvAdvance));
+  t_virt.flush(Ev(R, kEvVirtual));
+  t_cross.flush(Ev(R, kEvCross));
+  t_col.flush(Ev(R, kEvCollide));
+}
 #endif  // MMC_TRACKER_KERNELS
 
 // ThermalScattering::Scatter (ThermalScattering.cpp:159-171) with POD sampling.

@@ -736,7 +736,9 @@ static int BuildWorld(mmc_ctx* ctx, const mmc_world_desc* wd, const mmc_tally_de
   const size_t ev_static = std::max(
       {StaticLds(reinterpret_cast<const void*>(KAdvance<true, false>)),
        StaticLds(reinterpret_cast<const void*>(KAdvance<true, true>)),
-       StaticLds(reinterpret_cast<const void*>(KCollide<true>))});
+       StaticLds(reinterpret_cast<const void*>(KCollide<true>)),
+       StaticLds(reinterpret_cast<const void*>(KAdvCol<false>)),
+       StaticLds(reinterpret_cast<const void*>(KAdvCol<true>))});
   const size_t ev_static0 = std::max(
       {StaticLds(reinterpret_cast<const void*>(KAdvance<false, false>)),
        StaticLds(reinterpret_cast<const void*>(KAdvance<false, true>)),
@@ -933,6 +935,11 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
       em = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, KAdvance<true, true>, kBlock,
                                                         lds);
       ec = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, KCollide<true>, kBlock, lds);
+      int occ_f = 0;
+      if (ec == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, KAdvCol<true>, kBlock, lds) ==
+              hipSuccess)
+        occ_c = std::min(occ_c, occ_f);
     } else {
       ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a, KAdvance<false, false>, kBlock,
                                                         lds);
@@ -1166,12 +1173,27 @@ static void SwapBanks(mmc_ctx* ctx) {
 
 // move: KAdvance compacts the bank into ctx->alt (with the parked fallback
 // particles fb), and the banks are swapped before the rest of the iteration.
+// k-eigenvalue iterations with the tables in LDS run KAdvCol, which also
+// collides its particles; the others run KAdvance, then KCollide.
+static int LaunchTsl(mmc_ctx* ctx, int iter, int qin, uint32_t grid);
 static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid, bool move,
                            uint32_t* fb) {
   const size_t lds = ctx->ev_lds;
   const uint32_t sgrid = lds ? std::min(grid, ctx->lds_grid) : grid;
   ProfBegin(ctx, MMC_K_ADVANCE);
   const dim3 ag(sgrid), ab(kBlock);
+  if (ctx->R.keig && ctx->ev_tot_lds) {
+    // k-eigenvalue: the advance kernel collides its particles itself
+    if (move)
+      hipLaunchKernelGGL((KAdvCol<true>), ag, ab, lds, ctx->stream, ctx->W, ctx->R, iter, qin,
+                         ctx->alt, fb);
+    else
+      hipLaunchKernelGGL((KAdvCol<false>), ag, ab, lds, ctx->stream, ctx->W, ctx->R, iter, qin,
+                         ctx->R.B, nullptr);
+    ProfEnd(ctx);
+    if (move) SwapBanks(ctx);
+    return LaunchTsl(ctx, iter, qin, grid);
+  }
   if (ctx->ev_tot_lds) {
     if (move)
       hipLaunchKernelGGL((KAdvance<true, true>), ag, ab, lds, ctx->stream, ctx->W, ctx->R, iter,
@@ -1197,6 +1219,12 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid, bool 
     hipLaunchKernelGGL(KCollide<false>, dim3(sgrid), dim3(kBlock), lds, ctx->stream, ctx->W,
                        ctx->R, iter, qin);
   ProfEnd(ctx);
+  return LaunchTsl(ctx, iter, qin, grid);
+}
+
+// The thermal-scattering step of iteration `iter`: the network or the POD
+// sampler for the collisions queued in QT.
+static int LaunchTsl(mmc_ctx* ctx, int iter, int qin, uint32_t grid) {
   if (ctx->has_tsl) {
     if (ctx->opt.tsl_mode == MMC_TSL_SURROGATE && ctx->mlp.ready) {
       const int rc = MlpTslStep(ctx->mlp, ctx->W, ctx->R, iter, qin, grid, ctx->stream,
