@@ -637,8 +637,8 @@ __global__ void __launch_bounds__(256) KAdvance(DevWorld Wg, RunState R, int ite
     uint32_t iq, ic;
     // one atomic per block and pass: the block's collisions then sit together
     // in the collision queue, and the collision kernel reads neighbouring
-    // particle slots (KCollide 7.86 -> 7.50 ms per C5 cycle; KAdvance itself
-    // is unchanged)
+    // particle slots (KCollide 7.86 -> 7.50 ms per cycle of the multi-zone
+    // k-eigenvalue problem with 1e6 histories; KAdvance itself is unchanged)
     BlockPush2(Ring(R, iter, kQ), live, o.to_c, iq, ic);
     if (o.to_c) R.QC[ic] = slot;
     if (live) R.Q[qin ^ 1][iq] = slot;
@@ -996,10 +996,10 @@ __global__ void __launch_bounds__(256, 2) KCollide(DevWorld Wg, RunState R, int 
 
 // The advance and, where the flight ends in a real collision, the collision
 // of the same particle, in one kernel: KAdvance's work, then KCollide's on
-// the slot the advance just wrote (bank D when the kernel compacts).  In
-// C5 nine flights in ten end in a collision, so few lanes idle through the
-// collision code, and the launch, the collision queue and the collision
-// kernel's ramp go away.  k-eigenvalue iterations only: there a fission banks
+// the slot the advance just wrote (bank D when the kernel compacts).  In the
+// multi-zone k-eigenvalue problem nine flights in ten end in a collision, so
+// few lanes idle through the collision code, and the launch, the collision
+// queue and the collision kernel's ramp go away.  k-eigenvalue iterations only: there a fission banks
 // sites, while a fixed-source child claims a bank slot from misc[3], which
 // block 0 of this kernel sets.  Queues: alive after the flight (virtual
 // collision, crossing) -> Q (kQ); after the collision: thermal scattering ->
@@ -1119,7 +1119,8 @@ __global__ void __launch_bounds__(kThreads) KTslPod(DevWorld W, RunState R, int 
       p = R.QT[q];
       alive = TslPodEvent(W, R, p, H);
     }
-    // one atomic per block and pass (BlockPush): 5.92 -> 5.41 ms per C5 cycle
+    // one atomic per block and pass (BlockPush): 5.92 -> 5.41 ms per cycle of
+    // the multi-zone k-eigenvalue problem with 1e6 histories
     const uint32_t iq = BlockPush(Ring(R, iter, kQ), alive);
     if (alive) R.Q[qin ^ 1][iq] = p;
     t_tsl.add(valid);
