@@ -493,6 +493,16 @@ __device__ __forceinline__ bool CrossAt(const DevWorld& W, const RunState& R, ui
 struct AdvanceOut {
   bool to_x, to_c, to_q, x_alive;
 };
+// The particle as a flight that ends in a collision leaves it: the values
+// the advance writes to the bank, for a collision in the same kernel to take
+// from registers instead of reading them back.
+struct FlightEnd {
+  V3 pos, dir;
+  double E;
+  uint32_t g;
+  int cell;
+  int grid, gi;  // the grid-index hint
+};
 // Particle::Stream -> TotalCrossSectionPerturbation::Stream
 // (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
 // material that holds the perturbed nuclide, with sigma the collision
@@ -513,11 +523,13 @@ __device__ __noinline__ void StreamIndirect(const DevWorld& W, const RunState& R
 // written to slot i of bank D (every field a later event reads; the pending
 // thermal collision fields are dead here), not back to slot p: the advance
 // kernel compacts the bank as it goes.
+// keep: where a flight that ends in a real collision leaves its particle
+// (FlightEnd), or nullptr.
 template <bool kLocate, bool kMove>
 __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunState& R,
                                                    uint32_t p, const int* hash,
                                                    const double* Ht, const DevBank& D,
-                                                   uint32_t i) {
+                                                   uint32_t i, FlightEnd* keep = nullptr) {
   AdvanceOut o{false, false, false, false};
   const bool delta = W.tracking == 1;
   V3 pos = LoadPos(R.B, p);
@@ -578,6 +590,15 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     return o;
   }
   int to_cell = cell;
+  if (keep && o.to_c) {
+    keep->pos = pos;
+    keep->dir = dir;
+    keep->E = E;
+    keep->g = g;
+    keep->cell = cell;
+    keep->grid = memo.grid >= 0 ? memo.grid : memo.hint_grid;
+    keep->gi = memo.grid >= 0 ? memo.gi : memo.hint_gi;
+  }
   if (o.to_x) o.x_alive = CrossAt<kLocate, kMove>(W, R, p, pos, s, cell, dir, E, to_cell);
   if (kMove && (o.to_c || o.to_q || o.x_alive)) {
     D.x[i] = pos.x; D.y[i] = pos.y; D.z[i] = pos.z;
@@ -912,22 +933,24 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
 // Particle::Scatter).  `alive`: back to the live queue; `to_t`: a bound
 // thermal scattering whose (alpha, beta) is sampled by the TSL step.
 // Children of a fixed-source fission go to live_q / live_cnt.
-__device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& R, uint32_t p,
-                                             const int* hash, const double* Ht,
-                                             uint32_t* live_q, uint32_t* live_cnt, bool& alive,
-                                             bool& to_t) {
+// The collision of particle p in state f: the values the advance that queued
+// it left (in the bank, or in registers when the same kernel collides).
+__device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState& R, uint32_t p,
+                                               const FlightEnd& f, const int* hash,
+                                               const double* Ht, uint32_t* live_q,
+                                               uint32_t* live_cnt, bool& alive, bool& to_t) {
   alive = to_t = false;
-  const V3 pos = LoadPos(R.B, p);
-  V3 dir = LoadDir(R.B, p);
-  double E = R.B.E[p];
-  uint32_t g = R.B.rng[p];
-  const int cell = R.B.cell[p];
+  const V3 pos = f.pos;
+  V3 dir = f.dir;
+  double E = f.E;
+  uint32_t g = f.g;
+  const int cell = f.cell;
   const DCell c = W.cells[cell];
   const double T = FieldAt(W.fields[c.field], pos);
   XsMemo memo;
-  memo.hint_grid = R.B.xs_grid[p];
-  memo.hint_gi = R.B.xs_gi[p];
-  memo.hint_exact = true;  // written by the advance that queued this collision
+  memo.hint_grid = f.grid;
+  memo.hint_gi = f.gi;
+  memo.hint_exact = true;  // left by the advance that ended in this collision
   memo.tot = Ht;
   // the collision estimator of k: +0 increments (no fissile nuclide here) are
   // skipped, as for the track-length sum, and so is the total they would
@@ -961,6 +984,20 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
     R.B.E[p] = E;
   }
   if (alive || to_t) R.B.rng[p] = g;
+}
+__device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& R, uint32_t p,
+                                             const int* hash, const double* Ht,
+                                             uint32_t* live_q, uint32_t* live_cnt, bool& alive,
+                                             bool& to_t) {
+  FlightEnd f;
+  f.pos = LoadPos(R.B, p);
+  f.dir = LoadDir(R.B, p);
+  f.E = R.B.E[p];
+  f.g = R.B.rng[p];
+  f.cell = R.B.cell[p];
+  f.grid = R.B.xs_grid[p];
+  f.gi = R.B.xs_gi[p];
+  CollideEventAt(W, R, p, f, hash, Ht, live_q, live_cnt, alive, to_t);
 }
 
 #ifdef MMC_TRACKER_KERNELS
@@ -1027,9 +1064,10 @@ __global__ void __launch_bounds__(256, 2) KAdvCol(DevWorld Wg, RunState R, int i
     const bool valid = q < n;
     AdvanceOut o{false, false, false, false};
     uint32_t p = 0;
+    FlightEnd fe;
     if (valid) {
       p = LiveAt(R, qin, n1, q);
-      o = AdvanceEvent<true, kMove>(W, R, p, hash, Ht, D, q);
+      o = AdvanceEvent<true, kMove>(W, R, p, hash, Ht, D, q, &fe);
     } else if (kMove && q < n + nf) {
       const uint32_t f = fb[q - n];
       MoveParticle(R.B, f, D, q);
@@ -1039,7 +1077,7 @@ __global__ void __launch_bounds__(256, 2) KAdvCol(DevWorld Wg, RunState R, int i
     const uint32_t slot = kMove ? q : p;  // the particle's slot after this kernel
     const bool live = o.to_q || o.x_alive;
     bool alive = false, to_t = false;
-    if (o.to_c) CollideEvent(W, RD, slot, hash, Ht, nullptr, nullptr, alive, to_t);
+    if (o.to_c) CollideEventAt(W, RD, slot, fe, hash, Ht, nullptr, nullptr, alive, to_t);
     const uint32_t iq = BlockPush(Ring(R, iter, kQ), live);
     if (live) R.Q[qin ^ 1][iq] = slot;
     uint32_t it, i2;
