@@ -163,10 +163,16 @@ constexpr int kHStride = 264;      // bf16 per LDS activation row (256 + 8 paddi
 static_assert(kMlpWaves * 32 == kHid, "one wave per 32 output features");
 static_assert(kFRows % 64 == 0 && (64 * kMlpWaves) % kFRows == 0, "row tiling");
 // LDS: small parameters | X [kFRows] float4 | parts [waves][kFRows] float2 |
-// pidx [kFRows] | H0, H1 [kFRows][kHStride] bf16
+// pidx [kFRows] | H0, H1 [kFRows][kHStride] bf16 | each wave's layer-2 weight
+// fragments of k-blocks 8..15 [waves][8][64] bf16x8.  A block of 8 waves at
+// 256 VGPRs fills a CU, so the LDS beyond the activations is free; holding
+// half of the layer-2 weights there leaves the registers that the tracker's
+// FP64 finish needs.
+constexpr int kA2Regs = 8;  // k-blocks of the layer-2 weights held in registers
 constexpr size_t kFusedLds = sizeof(float) * (kSmall + 4 * kFRows + 2 * kMlpWaves * kFRows) +
                              sizeof(uint32_t) * kFRows +
-                             sizeof(__bf16) * 2 * kFRows * kHStride;
+                             sizeof(__bf16) * 2 * kFRows * kHStride +
+                             sizeof(bf16x8) * kMlpWaves * (16 - kA2Regs) * 64;
 static_assert(kFusedLds <= 160 * 1024, "LDS of one CU");
 
 // LDS position (B-fragment order) of the feature that accumulator register r
@@ -175,6 +181,26 @@ static_assert(kFusedLds <= 160 * 1024, "LDS of one CU");
 __host__ __device__ __forceinline__ int PosC(int t, int r, int h) {
   const int j = r / 4;
   return 16 * (2 * t + j / 2) + 8 * h + 4 * (j % 2) + r % 4;
+}
+// MfmaRows with the weights of k-blocks kA2Regs..15 read from LDS (this
+// wave's slice, As[(kb - kA2Regs) * 64 + lane]) and the others from A.
+__device__ __forceinline__ void MfmaRowsSplit(const __bf16* H, const bf16x8 A[kA2Regs],
+                                              const bf16x8* As, int lane, int h, int j,
+                                              f32x16 acc[kFCb]) {
+#pragma unroll
+  for (int c = 0; c < kFCb; c++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[c][r] = 0.0f;
+#pragma unroll
+  for (int kb = 0; kb < 16; kb++) {
+    const bf16x8 a = kb < kA2Regs ? A[kb] : As[(kb - kA2Regs) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < kFCb; c++) {
+      const bf16x8 b =
+          *reinterpret_cast<const bf16x8*>(H + (32 * c + j) * kHStride + 16 * kb + 8 * h);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[c], 0, 0, 0);
+    }
+  }
 }
 // acc[c] = A (this wave's 32 output features) x H rows 32c .. 32c + 31.
 __device__ __forceinline__ void MfmaRows(const __bf16* H, const bf16x8 A[16], int h, int j,
@@ -220,6 +246,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
   uint32_t* pidx = reinterpret_cast<uint32_t*>(parts + kMlpWaves * kFRows);
   __bf16* H0 = reinterpret_cast<__bf16*>(pidx + kFRows);
   __bf16* H1 = H0 + kFRows * kHStride;
+  bf16x8* A2s = reinterpret_cast<bf16x8*>(H1 + kFRows * kHStride);
   const uint32_t n = n_ptr ? *n_ptr : n_static;
   const uint32_t n_it = (n + kFRows - 1) / kFRows;
   if (blockIdx.x >= n_it) return;  // no rows: read nothing
@@ -229,11 +256,15 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
   const int j = lane % 32;
   for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
     reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(m.small)[i];
-  bf16x8 A1[16], A2[16];
+  // (a wave reads back only the LDS weights it wrote: no barrier)
+  bf16x8 A1[16], A2[kA2Regs];
+  bf16x8* const A2w = A2s + wave * (16 - kA2Regs) * 64;
 #pragma unroll
   for (int kb = 0; kb < 16; kb++) {
     A1[kb] = m.frag1[(wave * 16 + kb) * 64 + lane];
-    A2[kb] = m.frag2[(wave * 16 + kb) * 64 + lane];
+    const bf16x8 w2 = m.frag2[(wave * 16 + kb) * 64 + lane];
+    if (kb < kA2Regs) A2[kb] = w2;
+    else A2w[(kb - kA2Regs) * 64 + lane] = w2;
   }
   const float* bias1 = sp + kB1 + h * 128 + wave * 16;
   const float* bias2 = sp + kB2 + h * 128 + wave * 16;
@@ -337,7 +368,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
       }
     __syncthreads();
     // layer 2, then this wave's share of layer 3 (256 -> 2) in FP32
-    MfmaRows(H1, A2, h, j, acc);
+    MfmaRowsSplit(H1, A2, A2w, lane, h, j, acc);
 #pragma unroll
     for (int c = 0; c < kFCb; c++) {
       float o0 = 0.0f, o1 = 0.0f;
