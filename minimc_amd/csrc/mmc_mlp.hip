@@ -256,14 +256,17 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
   const int j = lane % 32;
   for (int i = threadIdx.x; i < kSmall / 4; i += blockDim.x)
     reinterpret_cast<float4*>(sp)[i] = reinterpret_cast<const float4*>(m.small)[i];
-  // (a wave reads back only the LDS weights it wrote: no barrier)
-  bf16x8 A1[16], A2[kA2Regs];
+  // The tracker keeps the layer-2 weights of k-blocks kA2Regs..15 in LDS (a
+  // wave reads back only the LDS weights it wrote: no barrier); batch
+  // inference, which has no FP64 finish, keeps them all in registers.
+  constexpr int kRegs2 = kTrack ? kA2Regs : 16;
+  bf16x8 A1[16], A2[kRegs2];
   bf16x8* const A2w = A2s + wave * (16 - kA2Regs) * 64;
 #pragma unroll
   for (int kb = 0; kb < 16; kb++) {
     A1[kb] = m.frag1[(wave * 16 + kb) * 64 + lane];
     const bf16x8 w2 = m.frag2[(wave * 16 + kb) * 64 + lane];
-    if (kb < kA2Regs) A2[kb] = w2;
+    if (kb < kRegs2) A2[kb] = w2;
     else A2w[(kb - kA2Regs) * 64 + lane] = w2;
   }
   const float* bias1 = sp + kB1 + h * 128 + wave * 16;
@@ -368,7 +371,10 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
       }
     __syncthreads();
     // layer 2, then this wave's share of layer 3 (256 -> 2) in FP32
-    MfmaRowsSplit(H1, A2, A2w, lane, h, j, acc);
+    if constexpr (kTrack)
+      MfmaRowsSplit(H1, A2, A2w, lane, h, j, acc);
+    else
+      MfmaRows(H1, A2, h, j, acc);
 #pragma unroll
     for (int c = 0; c < kFCb; c++) {
       float o0 = 0.0f, o1 = 0.0f;
