@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MMC_ABI_VERSION 2
+#define MMC_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum mmc_status {
@@ -308,6 +308,11 @@ typedef struct mmc_options {
                            park thermal collisions for the network; below
                            k the tail kernel samples them with POD.  0 (the
                            default): the tail samples with POD throughout */
+  int32_t split_collide; /* k-eigenvalue runs with the tables in LDS: 0 (the
+                           default): the advance kernel also does the
+                           collision that ends a flight; 1: separate
+                           advance and collision kernels, as in fixed-
+                           source runs.  Results are the same bit for bit. */
 } mmc_options;
 /* Reference behaviours that are reproduced only on request.
  * MMC_QUIRK_NUDGE_CROSSING: after a surface crossing the new cell is the

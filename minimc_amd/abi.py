@@ -159,7 +159,8 @@ class Options(C.Structure):
                 ("tail_threshold", C.c_int32), ("max_steps", C.c_uint64),
                 ("quirks", C.c_int32), ("compact_interval", C.c_int32),
                 ("graphs", C.c_int32), ("tail_block", C.c_int32),
-                ("tsl_block", C.c_int32), ("tail_network", C.c_int32)]
+                ("tsl_block", C.c_int32), ("tail_network", C.c_int32),
+                ("split_collide", C.c_int32)]
 
 
 class ParticleSoA(C.Structure):

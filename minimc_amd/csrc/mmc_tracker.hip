@@ -904,6 +904,10 @@ extern "C" int mmc_create(const mmc_world_desc* world, const mmc_tally_desc* tal
     *out = ctx;
     return Fail(ctx, MMC_ERR_ARG, "tail_network must be 0 or a particle count");
   }
+  if (ctx->opt.split_collide != 0 && ctx->opt.split_collide != 1) {
+    *out = ctx;
+    return Fail(ctx, MMC_ERR_ARG, "split_collide must be 0 or 1");
+  }
   ctx->tsl_block = ctx->opt.tsl_block ? ctx->opt.tsl_block : 256;
   if (hipSetDevice(ctx->device) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1182,7 +1186,7 @@ static int LaunchIteration(mmc_ctx* ctx, int iter, int qin, uint32_t grid, bool 
   const uint32_t sgrid = lds ? std::min(grid, ctx->lds_grid) : grid;
   ProfBegin(ctx, MMC_K_ADVANCE);
   const dim3 ag(sgrid), ab(kBlock);
-  if (ctx->R.keig && ctx->ev_tot_lds) {
+  if (ctx->R.keig && ctx->ev_tot_lds && !ctx->opt.split_collide) {
     // k-eigenvalue: the advance kernel collides its particles itself
     if (move)
       hipLaunchKernelGGL((KAdvCol<true>), ag, ab, lds, ctx->stream, ctx->W, ctx->R, iter, qin,

@@ -148,7 +148,8 @@ class Tracker:
 
     def __init__(self, problem, tracking=None, tsl_mode=abi.MMC_TSL_POD, device=0,
                  max_in_flight=0, graph_iters=0, max_steps=0, tail_threshold=0,
-                 compact_interval=0, graphs=0, tail_block=0, tsl_block=0, tail_network=0):
+                 compact_interval=0, graphs=0, tail_block=0, tsl_block=0, tail_network=0,
+                 split_collide=0):
         L = lib()
         self.problem = problem
         self._w, self._t, self._keep = problem.descriptors()
@@ -165,6 +166,7 @@ class Tracker:
         opt.tail_block = tail_block
         opt.tsl_block = tsl_block
         opt.tail_network = tail_network
+        opt.split_collide = split_collide
         self.options = opt
         h = C.c_void_p()
         rc = L.mmc_create(C.byref(self._w), C.byref(self._t), C.byref(opt), C.byref(h))

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
     L = C.CDLL(tracker.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
-    assert L.mmc_abi_version() == 2
+    assert L.mmc_abi_version() == 3
 
 
 def test_struct_sizes_match_c(tmp_path):

@@ -16,7 +16,10 @@ pytestmark = pytest.mark.gpu
 SCHEDULES = [
     dict(tail_threshold=-1, compact_interval=-1, graph_iters=8),   # events only
     dict(tail_threshold=1 << 30, compact_interval=-1, graph_iters=4),  # tail early
-    dict(tail_threshold=2048, compact_interval=1, graph_iters=8),  # compaction in KAdvance
+    dict(tail_threshold=2048, compact_interval=1, graph_iters=8),  # compaction in KAdvCol
+    # separate advance and collision kernels (KAdvance, KCollide)
+    dict(tail_threshold=2048, compact_interval=1, graph_iters=8, split_collide=1),
+    dict(tail_threshold=-1, compact_interval=2, graph_iters=8, split_collide=1),
     dict(tail_threshold=2048, compact_interval=2, graph_iters=8),  # KCompact
     dict(),                                                         # defaults
     # batches replayed as captured graphs (k-eigenvalue only)
