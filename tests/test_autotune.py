@@ -22,13 +22,15 @@ def test_search_picks_the_fastest(monkeypatch, tmp_path):
     monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "at.json"))
     best, table = autotune.tune(configs.c1_slab(), n=1000)
     assert best == {"tail_threshold": 16384, "graph_iters": 16, "compact_interval": 2}
-    # (no captured-graph candidate: compaction every 2 iterations; the tail
-    # block sizes, and no thermal-scattering block sizes: the slab has no
+    # (no captured-graph candidate: compaction every 2 iterations; separate
+    # advance and collision kernels (a k-eigenvalue problem); the tail block
+    # sizes, and no thermal-scattering block sizes: the slab has no
     # thermal-scattering law)
     assert len(table) == (len(autotune.TAIL_CANDIDATES) + len(autotune.ITER_CANDIDATES) - 1
-                          + len(autotune.COMPACT_CANDIDATES)
+                          + len(autotune.COMPACT_CANDIDATES) + 1
                           + len(autotune.TAIL_BLOCK_CANDIDATES))
-    assert [o for o, _ in table[-1:]] == [dict(best, tail_block=512)]
+    assert [o for o, _ in table[-2:]] == [dict(best, split_collide=1),
+                                          dict(best, tail_block=512)]
     # the cached result is returned without timing again
     monkeypatch.setattr(autotune, "_time_run", None)
     best2, _ = autotune.tune(configs.c1_slab(), n=1000)
