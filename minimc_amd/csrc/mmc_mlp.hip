@@ -164,11 +164,11 @@ static_assert(kMlpWaves * 32 == kHid, "one wave per 32 output features");
 static_assert(kFRows % 64 == 0 && (64 * kMlpWaves) % kFRows == 0, "row tiling");
 // LDS: small parameters | X [kFRows] float4 | parts [waves][kFRows] float2 |
 // pidx [kFRows] | H0, H1 [kFRows][kHStride] bf16 | each wave's layer-2 weight
-// fragments of k-blocks 8..15 [waves][8][64] bf16x8.  A block of 8 waves at
-// 256 VGPRs fills a CU, so the LDS beyond the activations is free; holding
-// half of the layer-2 weights there leaves the registers that the tracker's
-// FP64 finish needs.
-constexpr int kA2Regs = 8;  // k-blocks of the layer-2 weights held in registers
+// fragments of k-blocks kA2Regs..15 [waves][16 - kA2Regs][64] bf16x8.  A block
+// of 8 waves at 256 VGPRs fills a CU, so the LDS beyond the activations is
+// free; holding most of the layer-2 weights there leaves the registers that
+// the tracker's FP64 finish needs (7 in registers: no spill).
+constexpr int kA2Regs = 7;  // k-blocks of the layer-2 weights held in registers
 constexpr size_t kFusedLds = sizeof(float) * (kSmall + 4 * kFRows + 2 * kMlpWaves * kFRows) +
                              sizeof(uint32_t) * kFRows +
                              sizeof(__bf16) * 2 * kFRows * kHStride +
