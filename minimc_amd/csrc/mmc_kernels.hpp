@@ -777,7 +777,8 @@ struct CollideOut {
 // Multigroup.cpp:49-73,249-287, Continuous.cpp:57-70,
 // ContinuousReaction.cpp:60-62,119-189,252-265).  A thermal-scattering
 // scatter stops after choosing the reaction (out.tsl); the caller samples
-// (alpha, beta).
+// (alpha, beta).  k_abs: the absorption estimator's sum of this collision,
+// added to in k-eigenvalue runs (R->keig) only.
 __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, V3 pos,
                               V3& dir, double& E, uint32_t& g, int cell, uint32_t* live_q,
                               uint32_t* live_cnt, const int* hash, double* k_abs,
@@ -862,8 +863,8 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
         }
       }
     }
-    if (k_abs && (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) &&
-        cap + fi > 0)
+    if (k_abs && R && R->keig &&
+        (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) && cap + fi > 0)
       *k_abs += (W.mg_nubar[nuc * G + gi] * fi) / (cap + fi);
     return out;
   }
@@ -910,8 +911,8 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
           out.children++;
         }
       }
-      if (k_abs && (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) &&
-          n.fissile) {
+      if (k_abs && R && R->keig &&
+          (out.event == MMC_EV_CAPTURE || out.event == MMC_EV_FISSION) && n.fissile) {
         double sa = 0, nusf = 0;
         for (int q = 0; q < n.rx_count; q++) {
           const DCeReaction rr = W.rx[n.rx_begin + q];
@@ -963,8 +964,10 @@ __device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState
   }
   double kabs = 0;
   int tsl = -1;
-  const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, live_q, live_cnt, hash,
-                               R.keig ? &kabs : nullptr, &tsl, &memo);
+  // (kabs is summed only in k-eigenvalue runs; its address is passed either
+  // way, so that it stays in a register)
+  const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, live_q, live_cnt, hash, &kabs,
+                               &tsl, &memo);
   if (o.children < 0) atomicOr(R.misc, kErrReaction);
   if (R.keig) {
     if (kabs != 0) R.B.k_abs[R.B.hist[p]] += kabs;
