@@ -502,6 +502,10 @@ struct FlightEnd {
   uint32_t g;
   int cell;
   int grid, gi;  // the grid-index hint
+  // the advance's last thermal-scattering total (table tsl at (E, T)), or
+  // tsl -1: the collision's lookup at the same (E, T) takes it
+  int tsl;
+  double T, tsl_val;
 };
 // Particle::Stream -> TotalCrossSectionPerturbation::Stream
 // (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
@@ -590,7 +594,9 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     return o;
   }
   int to_cell = cell;
-  if (keep && o.to_c) {
+  if (o.to_x) {
+    o.x_alive = CrossAt<kLocate, kMove>(W, R, p, pos, s, cell, dir, E, to_cell);
+  } else if (keep && o.to_c) {
     keep->pos = pos;
     keep->dir = dir;
     keep->E = E;
@@ -598,8 +604,10 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     keep->cell = cell;
     keep->grid = memo.grid >= 0 ? memo.grid : memo.hint_grid;
     keep->gi = memo.grid >= 0 ? memo.gi : memo.hint_gi;
+    keep->tsl = memo.tsl;
+    keep->T = memo.tT;
+    keep->tsl_val = memo.tval;
   }
-  if (o.to_x) o.x_alive = CrossAt<kLocate, kMove>(W, R, p, pos, s, cell, dir, E, to_cell);
   if (kMove && (o.to_c || o.to_q || o.x_alive)) {
     D.x[i] = pos.x; D.y[i] = pos.y; D.z[i] = pos.z;
     D.u[i] = dir.x; D.v[i] = dir.y; D.w[i] = dir.z;
@@ -953,6 +961,11 @@ __device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState
   memo.hint_gi = f.gi;
   memo.hint_exact = true;  // left by the advance that ended in this collision
   memo.tot = Ht;
+  // the advance's thermal-scattering total at this energy (used where T matches)
+  memo.tsl = f.tsl;
+  memo.tE = E;
+  memo.tT = f.T;
+  memo.tval = f.tsl_val;
   // the collision estimator of k: +0 increments (no fissile nuclide here) are
   // skipped, as for the track-length sum, and so is the total they would
   // divide when no flux estimator needs it either
@@ -1000,6 +1013,8 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   f.cell = R.B.cell[p];
   f.grid = R.B.xs_grid[p];
   f.gi = R.B.xs_gi[p];
+  f.tsl = -1;
+  f.T = f.tsl_val = 0;
   CollideEventAt(W, R, p, f, hash, Ht, live_q, live_cnt, alive, to_t);
 }
 
