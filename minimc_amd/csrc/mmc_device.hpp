@@ -856,9 +856,13 @@ __device__ __forceinline__ double CeTotal(const DevWorld& W, const DCeNuclide& n
 }
 
 // Microscopic material total (or majorant): Material.cpp:41-61.
+// nt_nuc / nt_val (a total that is not the majorant, optional): the last
+// nuclide summed and its microscopic total CeTotal at (E, T).
 __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, double E,
                                                 double T, bool majorant,
-                                                const int* lds_hash, XsMemo* memo = nullptr) {
+                                                const int* lds_hash, XsMemo* memo = nullptr,
+                                                int* nt_nuc = nullptr,
+                                                double* nt_val = nullptr) {
   const DMaterial m = W.mats[mat];
   double acc = 0;
   if (W.energy_mode == 0) {
@@ -873,7 +877,12 @@ __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, doub
     const DComp c = W.comps[m.comp_begin + k];
     const DCeNuclide n = W.ce[c.nuclide];
     const int gi = NuclideGridIndex(W, n, lds_hash, E, memo);
-    acc = acc + c.afrac * CeTotal(W, n, gi, E, T, majorant, memo);
+    const double v = CeTotal(W, n, gi, E, T, majorant, memo);
+    if (nt_nuc && !majorant) {
+      *nt_nuc = c.nuclide;
+      *nt_val = v;
+    }
+    acc = acc + c.afrac * v;
   }
   return acc;
 }

@@ -502,10 +502,11 @@ struct FlightEnd {
   uint32_t g;
   int cell;
   int grid, gi;  // the grid-index hint
-  // the advance's last thermal-scattering total (table tsl at (E, T)), or
-  // tsl -1: the collision's lookup at the same (E, T) takes it
-  int tsl;
-  double T, tsl_val;
+  // the advance's last lookups at (E, T) that were not majorants: the
+  // thermal-scattering total of table tsl (or tsl -1) and nuclide nuc's
+  // total (or nuc -1); a collision at the same (E, T) takes them
+  int tsl, nuc;
+  double T, tsl_val, tot;
 };
 // Particle::Stream -> TotalCrossSectionPerturbation::Stream
 // (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
@@ -549,7 +550,12 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   memo.hint_grid = R.B.xs_grid[p];
   memo.hint_gi = R.B.xs_gi[p];
   memo.tot = Ht;
-  const double sigma = MaterialTotal(W, c.material, E, T, delta, hash, &memo);
+  // the temperature, nuclide and value of the last total that is not the
+  // majorant (for a collision that follows)
+  double T_nt = T;
+  int nt_nuc = -1;
+  double nt_val = 0;
+  const double sigma = MaterialTotal(W, c.material, E, T, delta, hash, &memo, &nt_nuc, &nt_val);
   const double d_coll = -log(1.0 - Canonical(g)) / (aden * sigma);
   int s;
   const double d_surf = NearestSurface(W, cell, pos, dir, s);
@@ -561,8 +567,9 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
   } else {
     bool real = true;
     if (delta) {
-      const double tot =
-          MaterialTotal(W, c.material, E, FieldAt(fld, pos), false, hash, &memo);
+      T_nt = FieldAt(fld, pos);
+      const double tot = MaterialTotal(W, c.material, E, T_nt, false, hash, &memo, &nt_nuc,
+                                       &nt_val);
       real = Canonical(g) < tot / sigma;  // pre-flight position (quirk Q6)
     }
     d = d_coll;
@@ -604,9 +611,11 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     keep->cell = cell;
     keep->grid = memo.grid >= 0 ? memo.grid : memo.hint_grid;
     keep->gi = memo.grid >= 0 ? memo.gi : memo.hint_gi;
-    keep->tsl = memo.tsl;
-    keep->T = memo.tT;
+    keep->T = T_nt;
+    keep->tsl = memo.tT == T_nt ? memo.tsl : -1;
     keep->tsl_val = memo.tval;
+    keep->nuc = nt_nuc;
+    keep->tot = nt_val;
   }
   if (kMove && (o.to_c || o.to_q || o.x_alive)) {
     D.x[i] = pos.x; D.y[i] = pos.y; D.z[i] = pos.z;
@@ -786,11 +795,13 @@ struct CollideOut {
 // ContinuousReaction.cpp:60-62,119-189,252-265).  A thermal-scattering
 // scatter stops after choosing the reaction (out.tsl); the caller samples
 // (alpha, beta).  k_abs: the absorption estimator's sum of this collision,
-// added to in k-eigenvalue runs (R->keig) only.
+// added to in k-eigenvalue runs (R->keig) only.  nt_nuc / nt_val: a nuclide
+// and its microscopic total CeTotal at this (E, T), known to the caller, or -1.
 __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, V3 pos,
                               V3& dir, double& E, uint32_t& g, int cell, uint32_t* live_q,
                               uint32_t* live_cnt, const int* hash, double* k_abs,
-                              int* tsl_index, XsMemo* memo = nullptr) {
+                              int* tsl_index, XsMemo* memo = nullptr, int nt_nuc = -1,
+                              double nt_val = 0) {
   CollideOut out{MMC_EV_SCATTER, 0, false};
   const DCell c = W.cells[cell];
   const DMaterial m = W.mats[c.material];
@@ -884,7 +895,8 @@ __device__ CollideOut Collide(const DevWorld& W, const RunState* R, uint32_t p, 
       out.children = -1;  // flags kErrReaction to the caller
       return out;
     }
-    const double threshold = Canonical(g) * CeTotal(W, n, gi, E, T, false, memo);
+    const double threshold =
+        Canonical(g) * (nuc == nt_nuc ? nt_val : CeTotal(W, n, gi, E, T, false, memo));
     double acc = 0;
     for (int k = 0; k < n.rx_count; k++) {
       const DCeReaction r = W.rx[n.rx_begin + k];
@@ -966,6 +978,8 @@ __device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState
   memo.tE = E;
   memo.tT = f.T;
   memo.tval = f.tsl_val;
+  // and its nuclide total, where T matches
+  const int nt_nuc = f.T == T ? f.nuc : -1;
   // the collision estimator of k: +0 increments (no fissile nuclide here) are
   // skipped, as for the track-length sum, and so is the total they would
   // divide when no flux estimator needs it either
@@ -980,7 +994,7 @@ __device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState
   // (kabs is summed only in k-eigenvalue runs; its address is passed either
   // way, so that it stays in a register)
   const CollideOut o = Collide(W, &R, p, pos, dir, E, g, cell, live_q, live_cnt, hash, &kabs,
-                               &tsl, &memo);
+                               &tsl, &memo, nt_nuc, f.tot);
   if (o.children < 0) atomicOr(R.misc, kErrReaction);
   if (R.keig) {
     if (kabs != 0) R.B.k_abs[R.B.hist[p]] += kabs;
@@ -1013,8 +1027,8 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   f.cell = R.B.cell[p];
   f.grid = R.B.xs_grid[p];
   f.gi = R.B.xs_gi[p];
-  f.tsl = -1;
-  f.T = f.tsl_val = 0;
+  f.tsl = f.nuc = -1;
+  f.T = f.tsl_val = f.tot = 0;
   CollideEventAt(W, R, p, f, hash, Ht, live_q, live_cnt, alive, to_t);
 }
 
