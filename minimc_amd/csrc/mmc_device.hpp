@@ -856,8 +856,9 @@ __device__ __forceinline__ double CeTotal(const DevWorld& W, const DCeNuclide& n
 }
 
 // Microscopic material total (or majorant): Material.cpp:41-61.
-// nt_nuc / nt_val (a total that is not the majorant, optional): the last
-// nuclide summed and its microscopic total CeTotal at (E, T).
+// nt_nuc / nt_val (a total that is not the majorant, optional): on entry a
+// nuclide whose microscopic total CeTotal at this (E, T) the caller knows
+// (nt_val), or -1; on return the last nuclide summed and its total.
 __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, double E,
                                                 double T, bool majorant,
                                                 const int* lds_hash, XsMemo* memo = nullptr,
@@ -877,7 +878,9 @@ __device__ __forceinline__ double MaterialTotal(const DevWorld& W, int mat, doub
     const DComp c = W.comps[m.comp_begin + k];
     const DCeNuclide n = W.ce[c.nuclide];
     const int gi = NuclideGridIndex(W, n, lds_hash, E, memo);
-    const double v = CeTotal(W, n, gi, E, T, majorant, memo);
+    const double v = nt_nuc && !majorant && *nt_nuc == c.nuclide
+                         ? *nt_val
+                         : CeTotal(W, n, gi, E, T, majorant, memo);
     if (nt_nuc && !majorant) {
       *nt_nuc = c.nuclide;
       *nt_val = v;

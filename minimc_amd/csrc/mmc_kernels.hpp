@@ -507,6 +507,7 @@ struct FlightEnd {
   // total (or nuc -1); a collision at the same (E, T) takes them
   int tsl, nuc;
   double T, tsl_val, tot;
+  double nuf;  // the material's nu Sigma_f at E (k-eigenvalue runs), or -1
 };
 // Particle::Stream -> TotalCrossSectionPerturbation::Stream
 // (Particle.cpp:46-53, Perturbation.cpp:68-78): 1 / sigma - distance in a
@@ -580,8 +581,10 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     StreamIndirect(W, R, p, W.mats[c.material].pert_mask, sigma, d);
   // k-eigenvalue sums: an increment of +0 (no fissile nuclide here) leaves
   // the non-negative sum as it is, so its read-modify-write is skipped
+  double nuf_e = -1;
   if (R.keig) {
     const double nuf = MaterialNuFission(W, c.material, E, hash, &memo);
+    nuf_e = nuf;
     if (nuf != 0) R.B.k_trk[R.B.hist[p]] += d * aden * nuf;
   }
   if (!kMove && memo.grid >= 0) {
@@ -616,6 +619,7 @@ __device__ __forceinline__ AdvanceOut AdvanceEvent(const DevWorld& W, const RunS
     keep->tsl_val = memo.tval;
     keep->nuc = nt_nuc;
     keep->tot = nt_val;
+    keep->nuf = nuf_e;
   }
   if (kMove && (o.to_c || o.to_q || o.x_alive)) {
     D.x[i] = pos.x; D.y[i] = pos.y; D.z[i] = pos.z;
@@ -983,9 +987,14 @@ __device__ __forceinline__ void CollideEventAt(const DevWorld& W, const RunState
   // the collision estimator of k: +0 increments (no fissile nuclide here) are
   // skipped, as for the track-length sum, and so is the total they would
   // divide when no flux estimator needs it either
-  const double nuf = R.keig ? MaterialNuFission(W, c.material, E, hash, &memo) : 0.0;
+  // (nu Sigma_f depends on the material and E only: the advance's value, where
+  // it has one, is this one)
+  const double nuf =
+      R.keig ? (f.nuf >= 0 ? f.nuf : MaterialNuFission(W, c.material, E, hash, &memo)) : 0.0;
   if (W.n_est_flux || nuf != 0) {
-    const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo);
+    int nt_in = nt_nuc;
+    double nt_v = f.tot;
+    const double tot = MaterialTotal(W, c.material, E, T, false, hash, &memo, &nt_in, &nt_v);
     if (W.n_est_flux) ScoreFlux(W, R, R.B.hist[p], pos, W.mats[c.material].aden * tot);
     if (nuf != 0) R.B.k_col[R.B.hist[p]] += nuf / tot;
   }
@@ -1029,6 +1038,7 @@ __device__ __forceinline__ void CollideEvent(const DevWorld& W, const RunState& 
   f.gi = R.B.xs_gi[p];
   f.tsl = f.nuc = -1;
   f.T = f.tsl_val = f.tot = 0;
+  f.nuf = -1;
   CollideEventAt(W, R, p, f, hash, Ht, live_q, live_cnt, alive, to_t);
 }
 
