@@ -1283,6 +1283,7 @@ __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int i
   uint32_t p = st == kAdv ? LiveAt(R, qin, n1, q) : 0u;
   unsigned long long c_adv = 0, c_virt = 0, c_cross = 0, c_coll = 0, c_tsl = 0, c_park = 0;
   unsigned long long c_tail = st == kAdv ? 1 : 0;
+  FlightEnd fe;  // a lane in state kCol: its particle, as its advance left it
   for (;;) {
     int pick = -1, most = 0;
 #pragma unroll
@@ -1296,7 +1297,7 @@ __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int i
     if (pick < 0) break;  // every lane of the wave is done
     if (st != pick) continue;
     if (st == kAdv) {
-      const AdvanceOut o = AdvanceEvent<false, false>(W, R, p, hash, Ht, R.B, p);
+      const AdvanceOut o = AdvanceEvent<false, false>(W, R, p, hash, Ht, R.B, p, &fe);
       c_adv++;
       c_virt += o.to_q ? 1 : 0;
       c_cross += o.to_x ? 1 : 0;
@@ -1304,7 +1305,7 @@ __global__ void __launch_bounds__(kThreads) KTail(DevWorld Wg, RunState R, int i
     } else if (st == kCol) {
       c_coll++;
       bool alive, to_t;
-      CollideEvent(W, R, p, hash, Ht, nullptr, nullptr, alive, to_t);
+      CollideEventAt(W, R, p, fe, hash, Ht, nullptr, nullptr, alive, to_t);
       if (kPark) {
         // one queue push for the wave's parked lanes (the lanes active in
         // this branch; the first of them adds the count)
