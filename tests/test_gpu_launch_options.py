@@ -41,19 +41,48 @@ def _cycles(pr, opts, n_cycles, cap):
     return out, c
 
 
+def _multizone_delta_linear_mixture(histories):
+    """The multi-zone problem with cell delta tracking (virtual collisions), a
+    linear temperature field in every cube (+-45 K about the cube's value, so a
+    collision's temperature differs from its flight's; inside the thermal-
+    scattering tables' 300-1200 K) and fuel mixed with
+    bound hydrogen (the nuclide a collision samples is not always the last one
+    summed into the material total)."""
+    import itertools
+    pr = configs.c5_multizone(histories=histories)
+    e = configs.C5_EDGES
+    g = (2.0, -1.5, 1.0)  # K/cm; 10 cm half-width: +-45 K
+    for n, (i, j, k) in enumerate(itertools.product(range(3), repeat=3)):
+        ctr = [(e[a] + e[a + 1]) / 2 for a in (i, j, k)]
+        c = min(max(pr.fields[n + 1]["c"], 345.0), 1155.0)
+        pr.fields[n + 1] = dict(kind=abi.MMC_FIELD_LINEAR, c=c, g=g,
+                                b=c - sum(gg * x for gg, x in zip(g, ctr)),
+                                lower=c - 45.0, upper=c + 45.0)
+    pr.materials[0]["comps"] = [(0, 0.8), (1, 0.2)]  # fuel + bound hydrogen
+    pr.tracking = abi.MMC_CELL_DELTA_TRACKING
+    return pr
+
+
 def _same_tallies(a, b, what):
     for e, ((S, S2), (T, T2)) in enumerate(zip(a, b)):
         assert np.array_equal(S, T), (what, e, np.flatnonzero(S != T)[:5])
         assert np.array_equal(S2, T2), (what, e, np.flatnonzero(S2 != T2)[:5])
 
 
-@pytest.mark.parametrize("case", ["multigroup_slab", "multizone"])
+@pytest.mark.parametrize("case", ["multigroup_slab", "multizone", "multizone_delta_linear"])
 def test_schedules_give_identical_cycles(case):
+    """Among the schedules, the joined advance-and-collision kernel and the
+    tail kernel hand the advance's lookups (thermal-scattering total, nuclide
+    total, nu Sigma_f) to the collision; the separate kernels (split_collide)
+    look everything up again.  The third case makes the collision's
+    temperature and sampled nuclide differ from the advance's."""
     require_gpu()
     if case == "multigroup_slab":
         pr, n_cycles, cap = configs.c1_slab(histories=4000), 6, 1 << 14
-    else:
+    elif case == "multizone":
         pr, n_cycles, cap = configs.c5_multizone(histories=20000), 4, 1 << 16
+    else:
+        pr, n_cycles, cap = _multizone_delta_linear_mixture(20000), 4, 1 << 16
     runs = [_cycles(pr, o, n_cycles, cap) for o in SCHEDULES]
     ref, ref_c = runs[0]
     for (st, c), opts in zip(runs[1:], SCHEDULES[1:]):
@@ -63,6 +92,8 @@ def test_schedules_give_identical_cycles(case):
         for k in ("collisions", "crossings", "advances", "tsl_collisions", "fission_sites"):
             assert c[k] == ref_c[k], (case, opts, k)
         _same_tallies(ref_c["tallies"], c["tallies"], (case, opts))
+    if case == "multizone_delta_linear":
+        assert ref_c["virtual_collisions"] > 0 and ref_c["tsl_collisions"] > 0
     # the early-tail schedule really ran histories in the tail kernel
     assert runs[1][1]["tail_particles"] > 0
     assert ref_c["tail_particles"] == 0
@@ -72,7 +103,7 @@ def test_schedules_give_identical_cycles(case):
     for (_, c), opts in zip(runs, SCHEDULES):
         if opts.get("graphs"):
             assert c["graph_launches"] >= c["graph_captures"], (case, opts, c)
-            if opts["tail_threshold"] < 0 or case == "multizone":
+            if opts["tail_threshold"] < 0 or case != "multigroup_slab":
                 assert c["graph_launches"] > c["graph_captures"] > 0, (case, opts, c)
         else:
             assert c["graph_launches"] == 0
